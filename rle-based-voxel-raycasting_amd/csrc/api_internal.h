@@ -1,0 +1,90 @@
+// Context and scene objects behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "device.h"
+#include "internal.h"
+
+namespace rlevox {
+
+// A device buffer that only grows.
+struct Scratch {
+  void* ptr = nullptr;
+  size_t size = 0;
+  rlevox_status ensure(size_t bytes);
+  void release();
+};
+
+rlevox_status hip_fail(hipError_t e, const char* what);
+
+inline bool tuning_valid(const rlevox_tuning& t) {
+  return (t.lines_per_wave == 16 || t.lines_per_wave == 32 || t.lines_per_wave == 64) &&
+         t.waves_per_block >= 1 && t.waves_per_block <= 4 && t.prefetch >= 0 && t.prefetch <= 4 &&
+         (t.order == 0 || t.order == 1);
+}
+
+inline int post_flags(const rlevox_render_params* p) {
+  return (p && p->post) ? p->post : (RLEVOX_POST_COMPOSE | RLEVOX_POST_GAPFILL);
+}
+
+inline size_t raybuf_bytes(const rlevox_raymap& rm) {
+  return (size_t)(rm.map_line_count > 0 ? rm.map_line_count : 1) * (size_t)rm.rs * 4;
+}
+inline size_t clip_bytes(const rlevox_raymap& rm) {
+  return (size_t)(rm.map_line_count > 0 ? rm.map_line_count : 1) * 8;
+}
+
+// A caller-supplied parameter block must keep every kernel loop bounded and
+// every access inside the buffers sized from it.
+inline rlevox_status check_raymap(const rlevox_raymap& rm) {
+  if (rm.W < 16 || rm.H < 16 || rm.H > rm.W || rm.W > RLEVOX_MAX_RS || rm.rs != rm.W ||
+      rm.maxres != rm.rs)
+    return fail(RLEVOX_ERR_INVALID_ARG, "raymap: bad size fields");
+  int64_t sum = 0;
+  for (int j = 0; j < 4; j++) {
+    if (rm.res[j] < 0 || rm.res[j] > 3 * rm.rs)
+      return fail(RLEVOX_ERR_INVALID_ARG, "raymap: res[%d] out of range", j);
+    sum += rm.res[j];
+  }
+  if (sum != rm.map_line_count) return fail(RLEVOX_ERR_INVALID_ARG, "raymap: line count mismatch");
+  if (rm.mapswitch < 1 || rm.mapswitch > (1 << 24) || rm.far_distance < 1 ||
+      rm.far_distance > (1 << 22))
+    return fail(RLEVOX_ERR_INVALID_ARG, "raymap: mip/far distance out of range");
+  return RLEVOX_OK;
+}
+
+rlevox_status ctx_bind(rlevox_ctx* ctx);
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t count, hipStream_t s);
+
+}  // namespace rlevox
+
+struct rlevox_ctx {
+  int device = 0;
+  int num_cus = 0;
+  char arch[64] = {0};
+  std::mutex mu;  // guards the scratch buffers and tuning
+  rlevox::Scratch raybuf, clip, ctmp;
+  rlevox_tuning tuning{64, 1, 0, 1};
+  int tuning_block() const { return tuning.lines_per_wave * tuning.waves_per_block; }
+  void release_all() {
+    raybuf.release();
+    clip.release();
+    ctmp.release();
+  }
+};
+
+struct rlevox_scene {
+  rlevox_ctx* ctx = nullptr;
+  int nmips = 0;
+  int sy[RLEVOX_MAX_MIPS] = {0};
+  rlevox::SceneDev dev;
+  std::vector<void*> allocs;
+  uint64_t bytes = 0;
+};

@@ -1,0 +1,469 @@
+// Device-side data layout and the per-line raycast routine shared by the
+// plain and batched kernels.
+//
+// Arithmetic contract (kept identical on every path so framebuffers are
+// bit-exact): fp32 in the reference's operation order, no FMA contraction
+// (-ffp-contract=off), IEEE division and square root, saturating float->int
+// conversion (v_cvt_i32_f32), NaN-ignoring fminf/fmaxf, and no
+// transcendental function on the device (the ray map carries them).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/rlevox.h"
+
+namespace rlevox {
+
+// One mip level in HBM: the pointer map as 8-byte entries, the slab array.
+struct MipDev {
+  const uint2* map;
+  const uint16_t* slabs;
+  int32_t sx, sz;
+};
+
+struct SceneDev {
+  MipDev mip[RLEVOX_MAX_MIPS];
+  int32_t nmips;
+  int32_t pad;
+};
+
+// Ray-buffer line bitmap words for render size rs.
+__host__ __device__ inline int bitmap_words(int rs) { return (rs + 31) >> 5; }
+
+__device__ inline int d_f2i(float f) {
+  if (f != f) return 0;
+  if (f >= 2147483648.0f) return INT32_MAX;
+  if (f < -2147483648.0f) return INT32_MIN;
+  return (int)f;
+}
+
+// LDS bitmap of one line, word w of this lane at base[w * stride].
+struct LineBits {
+  uint32_t* base;
+  int stride;
+  __device__ bool test(int y) const { return (base[(y >> 5) * stride] >> (y & 31)) & 1u; }
+  __device__ void set(int y) const { base[(y >> 5) * stride] |= 1u << (y & 31); }
+  __device__ void clear(int words) const {
+    for (int w = 0; w < words; w++) base[w * stride] = 0;
+  }
+};
+
+struct F3 {
+  float x, y, z;
+};
+__device__ inline F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+__device__ inline F3 f3ld(const float* p) { return F3{p[0], p[1], p[2]}; }
+__device__ inline F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline F3 operator*(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+
+// LineScale / ClipLine (Cuda_Render.h:39-65)
+__device__ inline float line_scale(F3 in, F3 ctr, float cmax, float cmin) {
+  float sx = 1, sy = 1;
+  if (ctr.x > 1) sx = (1 - in.x) / (ctr.x - in.x);
+  if (ctr.x < 0) sx = in.x / (in.x - ctr.x);
+  if (ctr.y > cmax) sy = (cmax - in.y) / (ctr.y - in.y);
+  if (ctr.y < cmin) sy = (-cmin + in.y) / (in.y - ctr.y);
+  return (sx < sy) ? sx : sy;
+}
+
+// MatMul (Cuda_Render.h:67-73), row-vector convention.
+__device__ inline F3 mat_apply(const float (*m)[4], F3 v) {
+  return f3(m[0][0] * v.x + m[1][0] * v.y + m[2][0] * v.z + m[3][0],
+            m[0][1] * v.x + m[1][1] * v.y + m[2][1] * v.z + m[3][1],
+            m[0][2] * v.x + m[1][2] * v.y + m[2][2] * v.z + m[3][2]);
+}
+
+// Raycasts ray line x of one view into its ray-buffer row and records the
+// cleared range in clip[0..1] (hi < lo when nothing was cleared).
+// Restates Render::render_line (Cuda_Render.h:96-736) with the active
+// configuration of core.h:24-32 (floating horizon, extended floating
+// horizon, LDS bitmap).
+__device__ inline void render_line(int x, const SceneDev& sc, const rlevox_raymap& rm,
+                                   uint32_t* __restrict__ row, int32_t* __restrict__ clip,
+                                   const LineBits bits) {
+  const int rs = rm.rs;
+  const float half = (float)(rs / 2);
+  clip[0] = 0;
+  clip[1] = -1;
+
+  // ---- line plane (Cuda_Render.h:128-172)
+  const int r0 = rm.res[0], r1 = r0 + rm.res[1], r2 = r1 + rm.res[2];
+  int q = 0;
+  if (x >= r2) q = 3;
+  else if (x >= r1) q = 2;
+  else if (x >= r0) q = 1;
+  float qofs = (float)x;
+  if (q == 1) qofs -= (float)r0;
+  if (q == 2) qofs -= (float)r1;
+  if (q == 3) qofs -= (float)r2;
+  const float a = qofs / (float)rm.res[q];
+  F3 p1 = f3ld(rm.p_2d[5]);
+  const F3 e0 = f3ld(rm.p_no[2 * q]);
+  F3 p2 = e0 + (f3ld(rm.p_no[2 * q + 1]) - e0) * a;
+  {
+    const float s2 = line_scale(p1, p2, rm.clip_max, rm.clip_min);
+    const F3 c2 = p1 + (p2 - p1) * s2;
+    const float s1 = line_scale(p2, p1, rm.clip_max, rm.clip_min);
+    const F3 c1 = p2 + (p1 - p2) * s1;
+    p1 = c1;
+    p2 = c2;
+  }
+  const F3 p1_3d = mat_apply(rm.to3d, p1 * 4.0f);
+  const F3 p2_3d = mat_apply(rm.to3d, p2 * 4.0f);
+  F3 dir = (p1_3d + p2_3d) * 0.5f - f3ld(rm.p4);
+  dir.y = 0;
+  {
+    const float d = dir.x * dir.x + dir.y * dir.y + dir.z * dir.z;
+    const float inv = 1.0f / sqrtf(d);
+    dir = dir * inv;
+  }
+  const float ray_x = rm.cos_myaw * dir.x + rm.sin_myaw * dir.z;
+  const float ray_z = rm.cos_myaw * dir.z - rm.sin_myaw * dir.x;
+  const bool vertical = q < 2;
+
+  const float sin_x = rm.sin_x, cos_x = rm.cos_x;
+  // ---- reverse texturing (Cuda_Render.h:203-210)
+  bool reverse = false;
+  if (vertical && ray_z <= 0) reverse = true;
+  if (!vertical && ray_x <= 0 && sin_x > 0) reverse = true;
+  if (!vertical && ray_x > 0 && sin_x < 0) reverse = true;
+  float hmul = reverse ? -half : half;
+  if (vertical) hmul = -hmul;
+
+  // ---- screen-space clip range (Cuda_Render.h:215-250)
+  const int p_add = reverse ? 1 : -2;
+  int p1x = d_f2i((float)rs * p1.x) + p_add;
+  int p1y = d_f2i((float)rs * p1.y) + p_add;
+  int p2x = d_f2i((float)rs * p2.x) - p_add;
+  int p2y = d_f2i((float)rs * p2.y) - p_add;
+  p1x = min(max(p1x, 0), rs - 1);
+  p1y = min(max(p1y, 0), rs - 1);
+  p2x = min(max(p2x, 0), rs - 1);
+  p2y = min(max(p2y, 0), rs - 1);
+  if (p1y == p2y) return;
+  int ymin, ymax;
+  if (vertical) {
+    ymin = rs - 1 - p1y;
+    ymax = rs - 1 - p2y;
+  } else {
+    ymin = rs - 1 - p1x;
+    ymax = rs - 1 - p2x;
+  }
+  if (reverse) {
+    ymin = rs - 1 - ymin;
+    ymax = rs - 1 - ymax;
+  }
+  if (ymin > ymax) {
+    const int t = ymin;
+    ymin = ymax;
+    ymax = t;
+  }
+  if (ymin >= ymax) return;
+
+  // ---- clear (Cuda_Render.h:255-264)
+  for (int n = ymin; n <= ymax; n++) row[n] = RLEVOX_CLEAR_TEXEL;
+  bits.clear(bitmap_words(rs));
+  clip[0] = ymin;
+  clip[1] = ymax;
+
+  // ---- DDA setup (Cuda_Render.h:270-304)
+  const float vpx = rm.position[0], vpy = rm.position[1], vpz = rm.position[2];
+  const float ddx = ray_x * rm.cos_y + ray_z * rm.sin_y;
+  const float ddy = ray_x * rm.sin_y - ray_z * rm.cos_y;
+  float fix_xf = -1, fix_yf = -1, sign_x = -1, sign_y = -1;
+  float frac_x = vpx - (float)d_f2i(vpx);
+  float frac_y = vpz - (float)d_f2i(vpz);
+  if (ddx >= 0) {
+    fix_xf = 0;
+    sign_x = 1;
+    frac_x = 1 - frac_x;
+  }
+  if (ddy >= 0) {
+    fix_yf = 0;
+    sign_y = 1;
+    frac_y = 1 - frac_y;
+  }
+  float g0y = ddy / fabsf(ddx), g0x = sign_x;
+  float g1x = ddx / fabsf(ddy), g1y = sign_y;
+  float i0x = g0x * frac_x, i0y = g0y * frac_x;
+  float i1x = g1x * frac_y, i1y = g1y * frac_y;
+  float gd0 = sqrtf(g0x * g0x + g0y * g0y);
+  float gd1 = sqrtf(g1x * g1x + g1y * g1y);
+  float dd0 = sqrtf(i0x * i0x + i0y * i0y);
+  float dd1 = sqrtf(i1x * i1x + i1y * i1y);
+  float pbx = 0, pby = 0, dbefore = 0;
+  float px = 0, py = 0, dnow = 0;
+  int idx = 0, idx_before = 0;
+
+  int mip = 0;
+  int gridx = sc.mip[0].sx, gridz = sc.mip[0].sz;
+  const uint2* map = sc.mip[0].map;
+  const uint16_t* slab = sc.mip[0].slabs;
+  const float z_add = sin_x;
+  const float y_add = (vertical ? cos_x : 0.0f) * hmul;
+  const float zfar = (float)rm.far_distance;
+  int mapswitch = rm.mapswitch;
+  float dz = 1, z = 0;
+  float yswitch = vpy;
+  const float mountain = vpy;
+
+  for (;;) {
+    // ---- distance mip switch (Cuda_Render.h:343-367)
+    while (z > (float)mapswitch || yswitch > 512.0f) {
+      yswitch = yswitch * 0.5f;
+      if (mip < sc.nmips - 1) {
+        mip++;
+        gridx >>= 1;
+        gridz >>= 1;
+        map = sc.mip[mip].map;
+        slab = sc.mip[mip].slabs;
+      }
+      g0x *= 2;
+      g0y *= 2;
+      g1x *= 2;
+      g1y *= 2;
+      gd0 *= 2;
+      gd1 *= 2;
+      mapswitch *= 2;
+      dz *= 2;
+    }
+    z += dz;
+    if (z > zfar) return;
+    if (ymin >= ymax) return;
+
+    // ---- DDA step (Cuda_Render.h:376-414)
+    dbefore = dnow;
+    pbx = px;
+    pby = py;
+    idx_before = idx;
+    if (dd1 < dd0) {
+      dnow = dd1;
+      idx = 1;
+      dd1 += gd1;
+      px = i1x;
+      py = i1y;
+      i1x += g1x;
+      i1y += g1y;
+    } else {
+      dnow = dd0;
+      idx = 0;
+      px = i0x;
+      py = i0y;
+      dd0 += gd0;
+      i0x += g0x;
+      i0y += g0y;
+    }
+    const int fix_x = (int)((float)(1 - idx_before) * fix_xf);
+    const int fix_z = (int)((float)idx_before * fix_yf);
+    const float ddelta = dnow - dbefore;
+    const float vs_x = ray_x * dbefore;
+    const float vs_z = ray_z * dbefore;
+    const int voxel_x = d_f2i(vpx + pbx) + fix_x;
+    const int voxel_z = d_f2i(vpz + pby) + fix_z;
+    const int vx = (voxel_x >> mip) & (gridx - 1);
+    const int vz = (voxel_z >> mip) & (gridz - 1);
+    const float corr_x = ray_x * ddelta;
+    const float corr_z = ray_z * ddelta;
+    const float pz = cos_x * vs_z + sin_x * mountain;
+    float py3 = vertical ? cos_x * mountain - sin_x * vs_z : vs_x;
+    py3 *= hmul;
+    // ---- clip top (Cuda_Render.h:467)
+    if (pz * half + py3 <= pz * (float)ymin) continue;
+
+    // ---- pointer-map gather (Cuda_Render.h:474-478)
+    const uint2 ent = map[vx + vz * gridx];
+    const uint32_t slen = ent.y & 0xffffu;
+    const float czz = cos_x * corr_z;
+    float cyy = vertical ? -sin_x * corr_z : corr_x;
+    cyy *= hmul;
+    const uint16_t* runs = slab + 2 + ent.x;
+    const uint16_t* colours = runs + slen;
+    uint32_t w = ent.y >> 16;
+    int ygen = 0;
+    float tex = 0;
+
+    // ---- RLE elements (Cuda_Render.h:511-734)
+    for (uint32_t k = 0; k < slen; k++) {
+      if (k > 0) w = runs[k];
+      int drawn = (int)(w >> 10);
+      ygen += (int)(w & 1023u) << mip;
+      if (drawn == 0) continue;
+      const int texture = (int)tex;
+      tex += (float)drawn;
+      drawn <<= mip;
+      const float ytop = (float)ygen;
+      ygen += drawn;
+      const float ybot = (float)ygen;
+
+      float zz1 = pz, yy1 = py3;
+      if (mountain + ytop >= 0) {
+        zz1 += czz;
+        yy1 += cyy;
+      }
+      const float pz1 = zz1 + z_add * ytop;
+      if (pz1 <= 0) continue;
+      const float py1 = yy1 + y_add * ytop;
+      int s2 = d_f2i(half + py1 / pz1);
+      if (s2 <= ymin) break;
+
+      float zz2 = pz, yy2 = py3;
+      if (mountain + ybot < 0) {
+        zz2 += czz;
+        yy2 += cyy;
+      }
+      const float pz2 = zz2 + z_add * ybot;
+      if (pz2 <= 0) continue;
+      const float py2 = yy2 + y_add * ybot;
+      int s1 = d_f2i(half + py2 / pz2 - 1.0f);
+      if (s1 >= ymax) continue;
+
+      // ---- floating horizon (Cuda_Render.h:563-580)
+      if (s2 >= ymax) {
+        s2 = ymax;
+        ymax = s1;
+      }
+      if (s1 <= ymin) {
+        s1 = ymin;
+        ymin = s2;
+        while (ymax > ymin && bits.test(ymin)) ++ymin;
+      }
+      int y = s1;
+      while (y < s2 && bits.test(y)) ++y;
+      if (y >= s2) continue;
+
+      // ---- perspective-correct span (Cuda_Render.h:642-733)
+      const float pz1r = pz + z_add * ytop;
+      const float py1r = py3 + y_add * ytop;
+      const float pz2r = pz + z_add * ybot;
+      const float py2r = py3 + y_add * ybot;
+      const float s2r = half + py1r / pz1r;
+      const float s1r = half + py2r / pz2r;
+      const float u1z = tex / pz2r;
+      float u2dz = (float)texture / pz1r - u1z;
+      const float onez1 = 1.0f / pz2r;
+      float onedz2 = 1.0f / pz1r - onez1;
+      u2dz /= s2r - s1r;
+      onedz2 /= s2r - s1r;
+      const float mult = (float)(y + 1) - s1r;
+      float uz = u1z + u2dz * mult;
+      float onez = onez1 + onedz2 * mult;
+      const float tlo = (float)texture, thi = tex - 1.0f;
+      for (; y < s2; ++y, uz += u2dz, onez += onedz2) {
+        if (bits.test(y)) continue;
+        const float fu = fminf(fmaxf(uz / onez, tlo), thi);
+        const uint32_t u = (uint32_t)d_f2i(fu);
+        const uint32_t depth = (uint32_t)d_f2i(1.0f / onez) & 0xfffeu;
+        row[y] = (uint32_t)colours[u] + (depth << 16);
+        bits.set(y);
+      }
+    }
+  }
+}
+
+// One screen pixel of the compose pass (colorize_buddha_soft.frag:12-92);
+// weights of exactly 0/1 are applied as selections.  Row 0 = image top.
+__device__ inline void compose_pixel(const rlevox_raymap& rm, const uint32_t* __restrict__ raybuf,
+                                     const int32_t* __restrict__ line_clip, int px, int prow,
+                                     uint16_t& out_c, int32_t& out_d) {
+  const float RESX = (float)rm.W, RESY = (float)rm.H;
+  const int rs = rm.rs;
+  const float border = (RESX - RESY) / (RESX * 2.0f);
+  const float vx = rm.vanish[0], vy = rm.vanish[1];
+  const float fcx = (float)px + 0.5f;
+  const float fcy = (float)(rm.H - 1 - prow) + 0.5f;
+  const float scx = fcx / RESX, scy = fcy / RESY;
+  const float scx1 = scx - vx, scy1 = scy - vy;
+  const bool upper = !(0.0f < scy1);
+  const bool left = !(0.0f < scx1);
+  const float e = fabsf(scy1) - fabsf(scx1) * RESX / RESY;
+  const bool ostep = !(0.0f < e);
+  const float o2 = (ostep ? fcx : fcy) / RESX;
+  const bool rxg = rm.rot_x_greater_zero != 0.0f;
+  float ty, tx;
+  if (!ostep) {
+    const float s = upper ? 1.0f : 0.0f;
+    float ang2 = scx1 * fabsf(1.0f - s - vy) / scy1;
+    ang2 = upper ? ang2 + (1.0f - vx) : ang2 + vx;
+    ty = upper ? (rm.ofs_add[1] + ang2) : (rm.ofs_add[0] + 1.0f - ang2);
+    const float ob = o2 + border;
+    const bool first = upper ? !rxg : rxg;
+    tx = first ? ob : 1.0f - ob;
+  } else {
+    const float t = left ? 1.0f : 0.0f;
+    float ang3 = scy1 * fabsf(1.0f - t - vx) / scx1;
+    ang3 = left ? ang3 + (1.0f - vy) : ang3 + vy;
+    ang3 = ang3 * RESY / RESX + border;
+    ty = left ? (rm.ofs_add[3] + ang3) : (rm.ofs_add[2] + 1.0f - ang3);
+    const bool first = left ? !rxg : rxg;
+    tx = first ? o2 : 1.0f - o2;
+  }
+  ty = ty * rm.ray_ratio * 0.25f;
+  uint16_t c = RLEVOX_SKY_COLOUR;
+  int32_t d = RLEVOX_SKY_DEPTH;
+  if (isfinite(tx) && isfinite(ty)) {
+    int tc = d_f2i(floorf(tx * (float)rs));
+    int tr = d_f2i(floorf(ty * (float)(4 * rs)));
+    tc = min(max(tc, 0), rs - 1);
+    tr = min(max(tr, 0), 4 * rs - 1);
+    if (tr < rm.map_line_count) {
+      const int2 lc = *reinterpret_cast<const int2*>(line_clip + 2 * tr);
+      if (tc >= lc.x && tc <= lc.y) {
+        const uint32_t t = raybuf[(size_t)tr * (size_t)rs + (size_t)tc];
+        if (((t >> 16) & 0xffu) != 0xffu) {
+          c = (uint16_t)(t & 0xffffu);
+          d = (int32_t)(t >> 16);
+        }
+      }
+    }
+  }
+  out_c = c;
+  out_d = d;
+}
+
+// Gap-fill / smoothing of one pixel (soft.frag:8-74 on integer buffers):
+// ring of 7 taps for the nearest depth, radius 78720/d in 1/256 px, 5x5 taps
+// within 1/0.7 of that depth averaged into the shade byte when more than 5
+// qualify.
+__constant__ const int kRingDx[7] = {0, 9, 9, 2, -8, -10, -3};
+__constant__ const int kRingDy[7] = {10, 6, -4, -10, -7, 3, 10};
+
+__device__ inline int floor_div_pos(int a, int b) {  // b > 0
+  const int q = a / b, r = a % b;
+  return (r != 0 && r < 0) ? q - 1 : q;
+}
+
+__device__ inline uint16_t gapfill_pixel(int W, int H, const uint16_t* __restrict__ cin,
+                                         const int32_t* __restrict__ dep, int x, int prow) {
+  const size_t i = (size_t)prow * W + x;
+  const uint16_t c0 = cin[i];
+  int dmin = dep[i];
+  for (int k = 0; k < 7; k++) {
+    const int xx = min(max(x + kRingDx[k], 0), W - 1);
+    const int yy = min(max(prow - kRingDy[k], 0), H - 1);
+    dmin = min(dmin, dep[(size_t)yy * W + xx]);
+  }
+  if (dmin >= RLEVOX_SKY_DEPTH) return c0;
+  const int r_q8 = 78720 / max(dmin, 1);
+  if (r_q8 < 42) return c0;
+  int n = 1, sum = c0 & 0xff;
+  for (int a = 0; a < 5; a++) {
+    const int ox = floor_div_pos((2 * a - 5) * r_q8 + 640, 1280);
+    const int xx = min(max(x + ox, 0), W - 1);
+    for (int b = 0; b < 5; b++) {
+      const int oy = floor_div_pos((2 * b - 5) * r_q8 + 640, 1280);
+      const int yy = min(max(prow - oy, 0), H - 1);
+      const size_t t = (size_t)yy * W + xx;
+      const int d = dep[t];
+      if (d < RLEVOX_SKY_DEPTH && (int64_t)10 * dmin >= (int64_t)7 * d) {
+        n++;
+        sum += cin[t] & 0xff;
+      }
+    }
+  }
+  if (n > 6) return (uint16_t)((c0 & 0xff00) | ((sum + n / 2) / n));
+  return c0;
+}
+
+}  // namespace rlevox

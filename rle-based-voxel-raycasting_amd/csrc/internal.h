@@ -1,0 +1,45 @@
+// Internal declarations shared by the host and device halves of librlevox.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <thread>
+#include <vector>
+#include <functional>
+
+#include "../../include/rlevox.h"
+
+namespace rlevox {
+
+// Records a message for rlevox_last_error() and returns `st`.
+rlevox_status fail(rlevox_status st, const char* fmt, ...);
+void clear_error();
+
+// Runs body(i) for i in [0, n) on `threads` host threads (0 = hardware
+// concurrency), in chunks handed out dynamically.
+void parallel_for(int64_t n, int threads, int64_t chunk, const std::function<void(int64_t)>& body);
+
+struct HostLevel {
+  int32_t sx = 0, sy = 0, sz = 0;
+  std::vector<uint32_t> ptrmap;
+  std::vector<uint16_t> slabs;
+};
+
+// Saturating float->int with NaN->0: the semantics of gfx950's v_cvt_i32_f32
+// and of CUDA's cvt.rzi.s32.f32, written out so host code agrees with it.
+inline int f2i(float f) {
+  if (f != f) return 0;
+  if (f >= 2147483648.0f) return INT32_MAX;
+  if (f < -2147483648.0f) return INT32_MIN;
+  return (int)f;
+}
+
+rlevox_status raymap_compute(const rlevox_pose& pose, int W, int H, int mip_distance,
+                             int far_distance, rlevox_raymap& out);
+
+}  // namespace rlevox
+
+struct rlevox_hscene {
+  int32_t nmips = 0;
+  rlevox::HostLevel lv[RLEVOX_MAX_MIPS];
+};

@@ -1,0 +1,90 @@
+// HIP kernels for gfx950: plain per-view raycast, compose, gap-fill, and the
+// batched persistent raycast over a device work queue.
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+#include "kernels.h"
+
+namespace rlevox {
+
+// ------------------------------------------------------------------ plain
+// One lane per ray line (cudaRender, Cuda_Main.cu:150-181); each lane keeps
+// its line bitmap in LDS, word w of lane l at lds[w*blockDim + l] so lanes
+// reading the same word index hit distinct banks.
+__global__ __launch_bounds__(256) void k_raycast_plain(SceneDev sc, rlevox_raymap rm,
+                                                       uint32_t* __restrict__ raybuf,
+                                                       int32_t* __restrict__ clip) {
+  extern __shared__ uint32_t lds[];
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= rm.map_line_count) return;
+  const LineBits bits{lds + threadIdx.x, (int)blockDim.x};
+  render_line(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs, clip + 2 * x, bits);
+}
+
+__global__ __launch_bounds__(256) void k_compose(rlevox_raymap rm, const uint32_t* __restrict__ raybuf,
+                                                 const int32_t* __restrict__ clip,
+                                                 uint16_t* __restrict__ col,
+                                                 int32_t* __restrict__ dep) {
+  const int px = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int row = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (px >= rm.W || row >= rm.H) return;
+  uint16_t c;
+  int32_t d;
+  compose_pixel(rm, raybuf, clip, px, row, c, d);
+  col[(size_t)row * rm.W + px] = c;
+  dep[(size_t)row * rm.W + px] = d;
+}
+
+__global__ __launch_bounds__(256) void k_gapfill(int W, int H, const uint16_t* __restrict__ cin,
+                                                 const int32_t* __restrict__ dep,
+                                                 uint16_t* __restrict__ cout) {
+  const int px = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int row = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (px >= W || row >= H) return;
+  cout[(size_t)row * W + px] = gapfill_pixel(W, H, cin, dep, px, row);
+}
+
+hipError_t launch_raycast_plain(const SceneDev& sc, const rlevox_raymap& rm, uint32_t* raybuf,
+                                int32_t* clip, int block, hipStream_t s) {
+  const int lines = rm.map_line_count;
+  if (lines <= 0) return hipSuccess;
+  const int grid = (lines + block - 1) / block;
+  const size_t lds = (size_t)bitmap_words(rm.rs) * block * sizeof(uint32_t);
+  hipLaunchKernelGGL(k_raycast_plain, dim3(grid), dim3(block), lds, s, sc, rm, raybuf, clip);
+  return hipGetLastError();
+}
+
+hipError_t launch_compose(const rlevox_raymap& rm, const uint32_t* raybuf, const int32_t* clip,
+                          uint16_t* col, int32_t* dep, hipStream_t s) {
+  dim3 grid((rm.W + 63) / 64, (rm.H + 3) / 4);
+  hipLaunchKernelGGL(k_compose, grid, dim3(256), 0, s, rm, raybuf, clip, col, dep);
+  return hipGetLastError();
+}
+
+hipError_t launch_gapfill(int W, int H, const uint16_t* cin, const int32_t* dep, uint16_t* cout,
+                          hipStream_t s) {
+  dim3 grid((W + 63) / 64, (H + 3) / 4);
+  hipLaunchKernelGGL(k_gapfill, grid, dim3(256), 0, s, W, H, cin, dep, cout);
+  return hipGetLastError();
+}
+
+}  // namespace rlevox
+
+namespace rlevox {
+
+// Fills count uint32 words; used instead of hipMemset so the same code can
+// be captured into a graph.
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint32_t v, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t count, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  uint64_t blocks = (count + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, value, count);
+  return hipGetLastError();
+}
+
+}  // namespace rlevox

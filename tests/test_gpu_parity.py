@@ -1,0 +1,135 @@
+"""Bit-exact parity of the HIP path with the oracle (GPU).
+
+Every stage is compared on the same inputs: the ray buffer (all texels,
+including the ones a line must leave untouched), the per-line cleared range,
+the composed framebuffer, the gap-filled framebuffer, and the whole-frame
+entry point.  Integer outputs must match exactly.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import rlevox  # noqa: E402
+import workloads  # noqa: E402
+from oracle import UNTOUCHED  # noqa: E402
+
+
+def gpu_stages(ctx, scene, rm):
+    lines = max(rm.map_line_count, 1)
+    d_rb = ctx.alloc(lines * rm.rs * 4)
+    d_clip = ctx.alloc(lines * 8)
+    d_rb.fill_u32(UNTOUCHED)
+    d_clip.fill_u32(0xFFFFFFFF)
+    ctx.render_raybuf(scene, rm, d_rb, d_clip)
+    d_col = ctx.alloc(rm.W * rm.H * 2)
+    d_dep = ctx.alloc(rm.W * rm.H * 4)
+    d_col2 = ctx.alloc(rm.W * rm.H * 2)
+    ctx.compose(rm, d_rb, d_clip, d_col, d_dep)
+    ctx.gapfill(rm.W, rm.H, d_col, d_dep, d_col2)
+    ctx.sync()
+    out = {
+        "raybuf": d_rb.download(np.uint32, lines * rm.rs).reshape(lines, rm.rs),
+        "clip": d_clip.download(np.int32, lines * 2).reshape(lines, 2),
+        "compose": d_col.download(np.uint16, rm.W * rm.H).reshape(rm.H, rm.W),
+        "depth": d_dep.download(np.int32, rm.W * rm.H).reshape(rm.H, rm.W),
+        "colour": d_col2.download(np.uint16, rm.W * rm.H).reshape(rm.H, rm.W),
+    }
+    for b in (d_rb, d_clip, d_col, d_dep, d_col2):
+        b.free()
+    return out
+
+
+def assert_same(name, got, want):
+    if not np.array_equal(got, want):
+        diff = np.argwhere(got != want)
+        i = tuple(diff[0])
+        raise AssertionError("%s: %d of %d differ; first at %s: gpu=%r oracle=%r"
+                             % (name, len(diff), got.size, i, got[i], want[i]))
+
+
+def check_view(ctx, oracle, dscene, mips, pose, W, H, mip_distance=0, far_distance=0):
+    rm = rlevox.raymap(pose, W, H, mip_distance, far_distance)
+    orm = oracle.raymap(pose, W, H, mip_distance, far_distance)
+    assert rm.to_bytes() == orm.to_bytes(), "ray map differs from the oracle's"
+    ref_rb, ref_clip, _ = oracle.raycast(mips, orm)
+    ref_col, ref_dep = oracle.compose(orm, ref_rb, ref_clip)
+    ref_fin = oracle.gapfill(ref_col, ref_dep)
+    got = gpu_stages(ctx, dscene, rm)
+    n = max(rm.map_line_count, 1)
+    assert_same("line clip", got["clip"][:n], ref_clip[:n])
+    assert_same("ray buffer", got["raybuf"], ref_rb)
+    assert_same("compose colour", got["compose"], ref_col)
+    assert_same("depth", got["depth"], ref_dep)
+    assert_same("gap-filled colour", got["colour"], ref_fin)
+    return rm
+
+
+@pytest.fixture(scope="module")
+def c1_dev(ctx, c1_scene):
+    s = ctx.upload(c1_scene.mips)
+    yield s
+    s.free()
+
+
+@pytest.fixture(scope="module")
+def tiny_dev(ctx, tiny_scene):
+    s = ctx.upload(tiny_scene.mips)
+    yield s
+    s.free()
+
+
+def test_c1_flythrough_512(ctx, oracle, c1_scene, c1_dev):
+    poses = workloads.flythrough(c1_scene.mips[0], frames=6, seed=3)
+    for p in poses:
+        check_view(ctx, oracle, c1_dev, c1_scene.mips, p, 512, 512)
+
+
+@pytest.mark.parametrize("W,H", [(640, 480), (1280, 720), (1920, 1080), (320, 240)])
+def test_c1_resolutions(ctx, oracle, c1_scene, c1_dev, W, H):
+    for p in workloads.random_views(c1_scene.mips[0], 2, seed=W + H):
+        check_view(ctx, oracle, c1_dev, c1_scene.mips, p, W, H)
+
+
+def test_c1_pitch_and_altitude_edges(ctx, oracle, c1_scene, c1_dev):
+    m0 = c1_scene.mips[0]
+    cases = [
+        (0.01, 0.3, 40.0), (-0.01, 1.0, 40.0), (-0.4, 2.0, 30.0),      # level and looking up
+        (1.2, 0.7, 80.0), (-1.2, 4.0, 20.0), (2.5, 5.5, 60.0),          # steep, over the top
+        (0.3, 0.0, 900.0), (0.3, math.pi / 2, 3000.0),                  # high: y mip switch
+        (0.2, 3.0, -30.0),                                               # inside the terrain
+    ]
+    for pitch, yaw, alt in cases:
+        x, z = 700.3, 800.6
+        p = workloads.pose(x, alt - workloads.surface_row(m0, x, z), z, pitch, yaw)
+        check_view(ctx, oracle, c1_dev, c1_scene.mips, p, 512, 384)
+
+
+def test_c1_mip_and_far_distance(ctx, oracle, c1_scene, c1_dev):
+    p = workloads.random_views(c1_scene.mips[0], 1, seed=99)[0]
+    for mipd, far in [(64, 5000), (300, 80000), (4096, 20000), (4, 100)]:
+        check_view(ctx, oracle, c1_dev, c1_scene.mips, p, 640, 480, mipd, far)
+
+
+def test_tiny_wrapping_world(ctx, oracle, tiny_scene, tiny_dev):
+    for p in workloads.random_views(tiny_scene.mips[0], 4, seed=5, alt=(2.0, 40.0)):
+        check_view(ctx, oracle, tiny_dev, tiny_scene.mips, p, 400, 300)
+
+
+def test_render_entry_matches_stages(ctx, oracle, c1_scene, c1_dev):
+    p = workloads.random_views(c1_scene.mips[0], 1, seed=1234)[0]
+    W, H = 800, 600
+    d_col = ctx.alloc(W * H * 2)
+    d_dep = ctx.alloc(W * H * 4)
+    ctx.render(c1_dev, p, W, H, d_col, d_dep)
+    col = d_col.download(np.uint16, W * H).reshape(H, W)
+    dep = d_dep.download(np.int32, W * H).reshape(H, W)
+    ref = oracle.render(c1_scene.mips, p, W, H)
+    assert_same("render colour", col, ref["colour"])
+    assert_same("render depth", dep, ref["depth"])
+    # compose only
+    ctx.render(c1_dev, p, W, H, d_col, d_dep, params=rlevox.RenderParams(0, 0, rlevox.POST_COMPOSE, 0))
+    col = d_col.download(np.uint16, W * H).reshape(H, W)
+    assert_same("render compose-only", col, ref["compose"])
