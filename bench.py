@@ -1,0 +1,248 @@
+"""Benchmark of the RLE voxel-column raycaster on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
+
+Workload (plain): a 4096x4096x1024 seeded terrain (8-octave value noise,
+~5 runs per column, mips 0-5) resident in HBM, rendered at 1920x1080 along a
+seeded 100-frame flythrough.  One step = one frame: ray map (host) + raycast
++ compose + gap-fill on the device; inputs are resident before timing starts.
+`value` is Mrays/s (one ray = one screen pixel) over the K timed frames.
+
+The roofline object is for the dominant kernel (the raycast): its algorithmic
+bytes per launch come from the exact work counts of every timed frame
+(8 B per pointer-map entry read, 2 B per extra RLE element, 2 B per colour,
+4 B per texel cleared and per texel written), divided by its mean duration
+from HIP events recorded on the launch stream around each launch.
+
+The batched object renders 64 views over 4 resident scenes of the same kind
+(resolutions uniform over 320x240, 1280x720, 1920x1080, 3840x2160) in one
+persistent launch and compares with 64 plain launches of the same views.
+
+The cpu_baseline object times the oracle (the CPU restatement, OpenMP over
+ray lines) on the host cores for a bounded sample of the same frames.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "rle-based-voxel-raycasting_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def raycast_bytes(c):
+    return (8 * c["ptr_loads"] + 2 * c["rle_loads"] + 2 * c["col_loads"]
+            + 4 * (c["cleared"] + c["col_loads"]))
+
+
+def post_bytes(W, H):
+    # compose: read one texel, write colour16 + depth32; gap-fill: read
+    # colour16 + depth32, write colour16 (taps hit cache).
+    return W * H * (4 + 2 + 4) + W * H * (2 + 4 + 2)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--seed", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(ray lines and scenes would shard across GPUs without collectives)", file=sys.stderr)
+        sys.exit(2)
+
+    import rlevox
+    import workloads
+
+    W, H = 1920, 1080
+    t0 = time.time()
+    hs = workloads.gen("c2", seed=args.seed)
+    log("generated c2 scene in %.1fs (%.2f GB host)" % (time.time() - t0, hs.nbytes / 1e9))
+    poses = workloads.flythrough(hs.mips[0], frames=100, seed=7)
+
+    ctx = rlevox.Context(0)
+    scene = ctx.upload(hs.mips)
+    log("uploaded %.2f GB to HBM" % (scene.device_bytes / 1e9))
+    stream = ctx.stream()
+    rms = [rlevox.raymap(p, W, H) for p in poses]
+    max_lines = max(r.map_line_count for r in rms)
+    d_rb = ctx.alloc(max_lines * W * 4)
+    d_clip = ctx.alloc(max_lines * 8)
+    d_c0 = ctx.alloc(W * H * 2)
+    d_col = ctx.alloc(W * H * 2)
+    d_dep = ctx.alloc(W * H * 4)
+
+    # exact work counts per pose (counting kernel, untimed)
+    counts = [ctx.raycast_counters(scene, rm, d_rb, d_clip) for rm in rms]
+    log("work counts done")
+
+    def frame(i, ev=None):
+        # the host ray map is part of every step
+        rm = rlevox.raymap(poses[i % len(poses)], W, H)
+        if ev is not None:
+            ctx.record(ev[0], stream)
+        ctx.render_raybuf(scene, rm, d_rb, d_clip, stream)
+        if ev is not None:
+            ctx.record(ev[1], stream)
+        ctx.compose(rm, d_rb, d_clip, d_c0, d_dep, stream)
+        ctx.gapfill(W, H, d_c0, d_dep, d_col, stream)
+        if ev is not None:
+            ctx.record(ev[2], stream)
+
+    for i in range(args.warmup):
+        frame(i)
+    ctx.sync(stream)
+    K = args.steps
+    evs = [(ctx.event(), ctx.event(), ctx.event()) for _ in range(K)]
+    ctx.sync(stream)
+    t0 = time.perf_counter()
+    for i in range(K):
+        frame(i, evs[i])
+    ctx.sync(stream)
+    wall = time.perf_counter() - t0
+    ray_ms = [ctx.elapsed_ms(e[0], e[1]) for e in evs]
+    post_ms = [ctx.elapsed_ms(e[1], e[2]) for e in evs]
+    ms_step = wall * 1000.0 / K
+    mrays = K * W * H / wall / 1e6
+    bytes_per = [raycast_bytes(counts[i % len(poses)]) for i in range(K)]
+    mean_bytes = float(np.mean(bytes_per))
+    mean_ray_ms = float(np.mean(ray_ms))
+    achieved = mean_bytes / (mean_ray_ms * 1e-3) / 1e9
+    log("plain: %.2f ms/frame, %.1f Mrays/s; raycast kernel mean %.3f ms (p50 %.3f, p99 %.3f), "
+        "post %.3f ms; %.1f MB algorithmic per raycast" % (
+            ms_step, mrays, mean_ray_ms, float(np.median(ray_ms)),
+            float(np.percentile(ray_ms, 99)), float(np.mean(post_ms)), mean_bytes / 1e6))
+
+    result = {
+        "metric": "Mrays/s @1080p plain vs batched; p50/p99 latency at fixed request rate",
+        "value": round(mrays, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+u16",
+        "data": "synthetic",
+        "config": {"workload": "plain 1920x1080 flythrough (100 seeded poses) over a resident "
+                               "4096x4096x1024 RLE terrain, mips 0-5 (config 2)",
+                   "scene_seed": args.seed, "pose_seed": 7, "resolution": [W, H],
+                   "frame_ms_p50": round(float(np.median(ray_ms) + np.median(post_ms)), 4),
+                   "raycast_ms_mean": round(mean_ray_ms, 4),
+                   "post_ms_mean": round(float(np.mean(post_ms)), 4)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": None, "kernel": "k_raycast_plain",
+                     "bytes_per_launch": round(mean_bytes)},
+    }
+
+    if not args.no_batch:
+        try:
+            result["batched"] = bench_batched(ctx, rlevox, workloads, args)
+        except Exception as e:  # reported, not fatal: the plain line stands
+            log("batched leg failed: %s" % e)
+            result["batched"] = {"error": str(e)}
+
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(hs, poses, W, H, args.cpu_seconds)
+
+    for b in (d_rb, d_clip, d_c0, d_col, d_dep):
+        b.free()
+    scene.free()
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
+def bench_batched(ctx, rlevox, workloads, args):
+    scenes_h = [workloads.gen("c2", seed=s) for s in (1, 2, 3, 4)]
+    log("generated 4 c2 scenes")
+    scenes = [ctx.upload(h.mips) for h in scenes_h]
+    views_spec = workloads.c4_views([h.mips[0] for h in scenes_h], n=64, seed=4)
+    bufs = []
+    views = []
+    rays = 0
+    for si, p, Wv, Hv in views_spec:
+        dc = ctx.alloc(Wv * Hv * 2)
+        dd = ctx.alloc(Wv * Hv * 4)
+        bufs += [dc, dd]
+        views.append(rlevox.make_view(scenes[si], p, Wv, Hv, dc, dd))
+        rays += Wv * Hv
+    stream = ctx.stream()
+    # plain: one launch sequence per view
+    for _ in range(2):
+        for v, (si, p, Wv, Hv) in zip(views, views_spec):
+            ctx.render(scenes[si], p, Wv, Hv, _Buf(v.d_colour), _Buf(v.d_depth), stream=stream)
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for v, (si, p, Wv, Hv) in zip(views, views_spec):
+            ctx.render(scenes[si], p, Wv, Hv, _Buf(v.d_colour), _Buf(v.d_depth), stream=stream)
+    ctx.sync(stream)
+    plain_s = (time.perf_counter() - t0) / reps
+    out = {"views": len(views), "scenes": 4, "rays_per_batch": rays,
+           "plain_mrays_s": round(rays / plain_s / 1e6, 2),
+           "plain_ms": round(plain_s * 1e3, 3)}
+    for graph in (False, True):
+        key = "graph" if graph else "persistent"
+        for _ in range(2):
+            ctx.render_batch(views, stream=stream, graph=graph)
+        ctx.sync(stream)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.render_batch(views, stream=stream, graph=graph)
+        ctx.sync(stream)
+        bs = (time.perf_counter() - t0) / reps
+        out[key + "_mrays_s"] = round(rays / bs / 1e6, 2)
+        out[key + "_ms"] = round(bs * 1e3, 3)
+    best = max(out.get("persistent_mrays_s", 0), out.get("graph_mrays_s", 0))
+    out["speedup_vs_plain"] = round(best / out["plain_mrays_s"], 3)
+    log("batched: %s" % out)
+    for b in bufs:
+        b.free()
+    for s in scenes:
+        s.free()
+    return out
+
+
+class _Buf:
+    def __init__(self, ptr):
+        import ctypes
+        self.ptr = ctypes.c_void_p(ptr)
+
+
+def cpu_baseline(hs, poses, W, H, seconds):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle
+    O = Oracle()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds and n < len(poses):
+        O.render(hs.mips, poses[n], W, H, threads=threads)
+        n += 1
+    dt = time.perf_counter() - t0
+    v = n * W * H / dt / 1e6
+    log("cpu baseline: %d frames in %.1fs on %d threads -> %.2f Mrays/s" % (n, dt, threads, v))
+    return {"value": round(v, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": "%d of the 100 flythrough frames at %dx%d (ray map + raycast + compose + "
+                      "gap-fill), OpenMP over ray lines" % (n, W, H)}
+
+
+if __name__ == "__main__":
+    main()

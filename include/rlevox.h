@@ -237,6 +237,14 @@ rlevox_status rlevox_render_raybuf(rlevox_ctx* ctx, const rlevox_scene* scene,
                                    const rlevox_raymap* rm, uint32_t* d_raybuf,
                                    int32_t* d_line_clip, void* stream);
 
+/* rlevox_render_raybuf plus the exact work counts of the frame (the
+ * DETAIL_BENCH counters, Cuda_Render.h:14-22 and Cuda_Main.cu:244-266);
+ * synchronous.  The counting kernel is a separate instantiation, so the
+ * plain entry point pays nothing for it. */
+rlevox_status rlevox_raycast_counters(rlevox_ctx* ctx, const rlevox_scene* scene,
+                                      const rlevox_raymap* rm, uint32_t* d_raybuf,
+                                      int32_t* d_line_clip, rlevox_counters* out);
+
 /* Compose (colorize_buddha_soft.frag:12-92): ray buffer -> W*H colour16 +
  * int32 depth, row 0 at the top of the image. */
 rlevox_status rlevox_compose(rlevox_ctx* ctx, const rlevox_raymap* rm, const uint32_t* d_raybuf,

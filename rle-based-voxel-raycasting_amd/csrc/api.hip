@@ -160,6 +160,26 @@ rlevox_status rlevox_render_raybuf(rlevox_ctx* ctx, const rlevox_scene* scene,
   return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "raycast launch");
 }
 
+rlevox_status rlevox_raycast_counters(rlevox_ctx* ctx, const rlevox_scene* scene,
+                                      const rlevox_raymap* rm, uint32_t* d_raybuf,
+                                      int32_t* d_line_clip, rlevox_counters* out) {
+  clear_error();
+  rlevox_status st = ctx_bind(ctx);
+  if (st != RLEVOX_OK) return st;
+  if (!scene || !rm || !d_raybuf || !d_line_clip || !out)
+    return fail(RLEVOX_ERR_INVALID_ARG, "raycast_counters: null argument");
+  st = check_raymap(*rm);
+  if (st != RLEVOX_OK) return st;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if ((st = ctx->counts.ensure(sizeof(rlevox_counters))) != RLEVOX_OK) return st;
+  unsigned long long* d = (unsigned long long*)ctx->counts.ptr;
+  hipError_t e = launch_fill_u32((uint32_t*)d, 0, sizeof(rlevox_counters) / 4, nullptr);
+  if (e == hipSuccess)
+    e = launch_raycast_plain(scene->dev, *rm, d_raybuf, d_line_clip, ctx->tuning_block(), nullptr, d);
+  if (e == hipSuccess) e = hipMemcpy(out, d, sizeof(rlevox_counters), hipMemcpyDeviceToHost);
+  return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "raycast_counters");
+}
+
 rlevox_status rlevox_compose(rlevox_ctx* ctx, const rlevox_raymap* rm, const uint32_t* d_raybuf,
                              const int32_t* d_line_clip, uint16_t* d_colour, int32_t* d_depth,
                              void* stream) {

@@ -70,13 +70,14 @@ struct rlevox_ctx {
   int num_cus = 0;
   char arch[64] = {0};
   std::mutex mu;  // guards the scratch buffers and tuning
-  rlevox::Scratch raybuf, clip, ctmp;
+  rlevox::Scratch raybuf, clip, ctmp, counts;
   rlevox_tuning tuning{64, 1, 0, 1};
   int tuning_block() const { return tuning.lines_per_wave * tuning.waves_per_block; }
   void release_all() {
     raybuf.release();
     clip.release();
     ctmp.release();
+    counts.release();
   }
 };
 

@@ -75,14 +75,23 @@ __device__ inline F3 mat_apply(const float (*m)[4], F3 v) {
             m[0][2] * v.x + m[1][2] * v.y + m[2][2] * v.z + m[3][2]);
 }
 
+// Exact work counts of one line (the DETAIL_BENCH counters of
+// Cuda_Render.h:14-22, extended); only the counting kernel variant keeps them.
+struct LineCounts {
+  uint64_t steps = 0, ptr = 0, rle = 0, elems = 0, spans = 0, col = 0, cleared = 0;
+};
+
 // Raycasts ray line x of one view into its ray-buffer row and records the
 // cleared range in clip[0..1] (hi < lo when nothing was cleared).
 // Restates Render::render_line (Cuda_Render.h:96-736) with the active
 // configuration of core.h:24-32 (floating horizon, extended floating
-// horizon, LDS bitmap).
+// horizon, LDS bitmap).  With COUNT the work counts are added to `counts`
+// (rlevox_counters layout) at the end of the line.
+template <bool COUNT>
 __device__ inline void render_line(int x, const SceneDev& sc, const rlevox_raymap& rm,
                                    uint32_t* __restrict__ row, int32_t* __restrict__ clip,
-                                   const LineBits bits) {
+                                   const LineBits bits, unsigned long long* counts = nullptr) {
+  LineCounts cnt;
   const int rs = rm.rs;
   const float half = (float)(rs / 2);
   clip[0] = 0;
@@ -167,6 +176,7 @@ __device__ inline void render_line(int x, const SceneDev& sc, const rlevox_rayma
   bits.clear(bitmap_words(rs));
   clip[0] = ymin;
   clip[1] = ymax;
+  if (COUNT) cnt.cleared = (uint64_t)(ymax - ymin + 1);
 
   // ---- DDA setup (Cuda_Render.h:270-304)
   const float vpx = rm.position[0], vpy = rm.position[1], vpz = rm.position[2];
@@ -230,8 +240,9 @@ __device__ inline void render_line(int x, const SceneDev& sc, const rlevox_rayma
       dz *= 2;
     }
     z += dz;
-    if (z > zfar) return;
-    if (ymin >= ymax) return;
+    if (z > zfar) break;
+    if (ymin >= ymax) break;
+    if (COUNT) cnt.steps++;
 
     // ---- DDA step (Cuda_Render.h:376-414)
     dbefore = dnow;
@@ -274,6 +285,7 @@ __device__ inline void render_line(int x, const SceneDev& sc, const rlevox_rayma
 
     // ---- pointer-map gather (Cuda_Render.h:474-478)
     const uint2 ent = map[vx + vz * gridx];
+    if (COUNT) cnt.ptr++;
     const uint32_t slen = ent.y & 0xffffu;
     const float czz = cos_x * corr_z;
     float cyy = vertical ? -sin_x * corr_z : corr_x;
@@ -287,6 +299,10 @@ __device__ inline void render_line(int x, const SceneDev& sc, const rlevox_rayma
     // ---- RLE elements (Cuda_Render.h:511-734)
     for (uint32_t k = 0; k < slen; k++) {
       if (k > 0) w = runs[k];
+      if (COUNT) {
+        cnt.elems++;
+        if (k > 0) cnt.rle++;
+      }
       int drawn = (int)(w >> 10);
       ygen += (int)(w & 1023u) << mip;
       if (drawn == 0) continue;
@@ -349,6 +365,7 @@ __device__ inline void render_line(int x, const SceneDev& sc, const rlevox_rayma
       const float mult = (float)(y + 1) - s1r;
       float uz = u1z + u2dz * mult;
       float onez = onez1 + onedz2 * mult;
+      if (COUNT) cnt.spans++;
       const float tlo = (float)texture, thi = tex - 1.0f;
       for (; y < s2; ++y, uz += u2dz, onez += onedz2) {
         if (bits.test(y)) continue;
@@ -357,8 +374,19 @@ __device__ inline void render_line(int x, const SceneDev& sc, const rlevox_rayma
         const uint32_t depth = (uint32_t)d_f2i(1.0f / onez) & 0xfffeu;
         row[y] = (uint32_t)colours[u] + (depth << 16);
         bits.set(y);
+        if (COUNT) cnt.col++;
       }
     }
+  }
+  if (COUNT) {
+    atomicAdd(counts + 0, 1ull);
+    atomicAdd(counts + 1, (unsigned long long)cnt.steps);
+    atomicAdd(counts + 2, (unsigned long long)cnt.ptr);
+    atomicAdd(counts + 3, (unsigned long long)cnt.rle);
+    atomicAdd(counts + 4, (unsigned long long)cnt.elems);
+    atomicAdd(counts + 5, (unsigned long long)cnt.spans);
+    atomicAdd(counts + 6, (unsigned long long)cnt.col);
+    atomicAdd(counts + 7, (unsigned long long)cnt.cleared);
   }
 }
 

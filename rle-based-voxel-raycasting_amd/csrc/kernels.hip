@@ -11,14 +11,16 @@ namespace rlevox {
 // One lane per ray line (cudaRender, Cuda_Main.cu:150-181); each lane keeps
 // its line bitmap in LDS, word w of lane l at lds[w*blockDim + l] so lanes
 // reading the same word index hit distinct banks.
+template <bool COUNT>
 __global__ __launch_bounds__(256) void k_raycast_plain(SceneDev sc, rlevox_raymap rm,
                                                        uint32_t* __restrict__ raybuf,
-                                                       int32_t* __restrict__ clip) {
+                                                       int32_t* __restrict__ clip,
+                                                       unsigned long long* counts) {
   extern __shared__ uint32_t lds[];
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= rm.map_line_count) return;
   const LineBits bits{lds + threadIdx.x, (int)blockDim.x};
-  render_line(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs, clip + 2 * x, bits);
+  render_line<COUNT>(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs, clip + 2 * x, bits, counts);
 }
 
 __global__ __launch_bounds__(256) void k_compose(rlevox_raymap rm, const uint32_t* __restrict__ raybuf,
@@ -45,12 +47,18 @@ __global__ __launch_bounds__(256) void k_gapfill(int W, int H, const uint16_t* _
 }
 
 hipError_t launch_raycast_plain(const SceneDev& sc, const rlevox_raymap& rm, uint32_t* raybuf,
-                                int32_t* clip, int block, hipStream_t s) {
+                                int32_t* clip, int block, hipStream_t s,
+                                unsigned long long* counts) {
   const int lines = rm.map_line_count;
   if (lines <= 0) return hipSuccess;
   const int grid = (lines + block - 1) / block;
   const size_t lds = (size_t)bitmap_words(rm.rs) * block * sizeof(uint32_t);
-  hipLaunchKernelGGL(k_raycast_plain, dim3(grid), dim3(block), lds, s, sc, rm, raybuf, clip);
+  if (counts)
+    hipLaunchKernelGGL(k_raycast_plain<true>, dim3(grid), dim3(block), lds, s, sc, rm, raybuf,
+                       clip, counts);
+  else
+    hipLaunchKernelGGL(k_raycast_plain<false>, dim3(grid), dim3(block), lds, s, sc, rm, raybuf,
+                       clip, counts);
   return hipGetLastError();
 }
 

@@ -143,6 +143,7 @@ _sigs = {
     "rlevox_scene_free": (None, [_vp]),
     "rlevox_scene_device_bytes": (C.c_uint64, [_vp]),
     "rlevox_render_raybuf": (C.c_int, [_vp, _vp, P(RayMap), _vp, _vp, _vp]),
+    "rlevox_raycast_counters": (C.c_int, [_vp, _vp, P(RayMap), _vp, _vp, P(Counters)]),
     "rlevox_compose": (C.c_int, [_vp, P(RayMap), _vp, _vp, _vp, _vp, _vp]),
     "rlevox_gapfill": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp, _vp, _vp, _vp]),
     "rlevox_render": (C.c_int, [_vp, _vp, P(Pose), C.c_int32, C.c_int32, P(RenderParams), _vp, _vp, _vp]),
@@ -394,6 +395,12 @@ class Context:
     def render_raybuf(self, scene, rm, d_raybuf, d_clip, stream=None):
         check(lib.rlevox_render_raybuf(self.handle, scene.handle, C.byref(rm), d_raybuf.ptr,
                                        d_clip.ptr, stream))
+
+    def raycast_counters(self, scene, rm, d_raybuf, d_clip):
+        c = Counters()
+        check(lib.rlevox_raycast_counters(self.handle, scene.handle, C.byref(rm), d_raybuf.ptr,
+                                          d_clip.ptr, C.byref(c)))
+        return c.as_dict()
 
     def compose(self, rm, d_raybuf, d_clip, d_col, d_dep, stream=None):
         check(lib.rlevox_compose(self.handle, C.byref(rm), d_raybuf.ptr, d_clip.ptr, d_col.ptr,

@@ -133,3 +133,17 @@ def test_render_entry_matches_stages(ctx, oracle, c1_scene, c1_dev):
     ctx.render(c1_dev, p, W, H, d_col, d_dep, params=rlevox.RenderParams(0, 0, rlevox.POST_COMPOSE, 0))
     col = d_col.download(np.uint16, W * H).reshape(H, W)
     assert_same("render compose-only", col, ref["compose"])
+
+
+def test_work_counters_match_oracle(ctx, oracle, c1_scene, c1_dev):
+    """The counting kernel's exact work counts equal the oracle's."""
+    for p in workloads.flythrough(c1_scene.mips[0], frames=3, seed=11):
+        rm = rlevox.raymap(p, 1280, 720)
+        lines = max(rm.map_line_count, 1)
+        d_rb = ctx.alloc(lines * rm.rs * 4)
+        d_clip = ctx.alloc(lines * 8)
+        got = ctx.raycast_counters(c1_dev, rm, d_rb, d_clip)
+        _, _, want = oracle.raycast(c1_scene.mips, oracle.raymap(p, 1280, 720))
+        assert got == want
+        d_rb.free()
+        d_clip.free()
