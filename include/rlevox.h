@@ -158,7 +158,9 @@ typedef struct {
 /* Tuning knobs of the raycast kernels (autotuner output).  Every value gives
  * bit-identical framebuffers; only speed changes. */
 typedef struct {
-  int32_t lines_per_wave;  /* 16, 32 or 64 lines mapped to one wavefront */
+  int32_t lines_per_wave;  /* 1 = one wavefront walks each line (lanes take
+                              consecutive DDA steps); 16, 32 or 64 = one lane
+                              per line */
   int32_t waves_per_block; /* 1..4 */
   int32_t prefetch;        /* pointer-map entries fetched ahead per line (0..4) */
   int32_t order;           /* 0 = view order, 1 = longest views first */

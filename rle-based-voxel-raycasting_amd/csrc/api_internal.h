@@ -24,7 +24,8 @@ struct Scratch {
 rlevox_status hip_fail(hipError_t e, const char* what);
 
 inline bool tuning_valid(const rlevox_tuning& t) {
-  return (t.lines_per_wave == 16 || t.lines_per_wave == 32 || t.lines_per_wave == 64) &&
+  return (t.lines_per_wave == 1 || t.lines_per_wave == 16 || t.lines_per_wave == 32 ||
+          t.lines_per_wave == 64) &&
          t.waves_per_block >= 1 && t.waves_per_block <= 4 && t.prefetch >= 0 && t.prefetch <= 4 &&
          (t.order == 0 || t.order == 1);
 }
@@ -71,8 +72,11 @@ struct rlevox_ctx {
   char arch[64] = {0};
   std::mutex mu;  // guards the scratch buffers and tuning
   rlevox::Scratch raybuf, clip, ctmp, counts;
-  rlevox_tuning tuning{64, 1, 0, 1};
-  int tuning_block() const { return tuning.lines_per_wave * tuning.waves_per_block; }
+  rlevox_tuning tuning{1, 1, 0, 1};
+  // 1 selects the wavefront-per-line kernel; otherwise lanes per block.
+  int tuning_block() const {
+    return tuning.lines_per_wave == 1 ? 1 : tuning.lines_per_wave * tuning.waves_per_block;
+  }
   void release_all() {
     raybuf.release();
     clip.release();

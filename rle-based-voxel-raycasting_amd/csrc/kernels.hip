@@ -4,6 +4,7 @@
 
 #include "device.h"
 #include "kernels.h"
+#include "wave_line.h"
 
 namespace rlevox {
 
@@ -21,6 +22,18 @@ __global__ __launch_bounds__(256) void k_raycast_plain(SceneDev sc, rlevox_rayma
   if (x >= rm.map_line_count) return;
   const LineBits bits{lds + threadIdx.x, (int)blockDim.x};
   render_line<COUNT>(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs, clip + 2 * x, bits, counts);
+}
+
+// One wavefront (= one block) per ray line.
+template <bool COUNT>
+__global__ __launch_bounds__(64) void k_raycast_wave(SceneDev sc, rlevox_raymap rm,
+                                                     uint32_t* __restrict__ raybuf,
+                                                     int32_t* __restrict__ clip,
+                                                     unsigned long long* counts) {
+  extern __shared__ uint32_t lds[];
+  const int x = blockIdx.x;
+  if (x >= rm.map_line_count) return;
+  render_line_wave<COUNT>(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs, clip + 2 * x, lds, counts);
 }
 
 __global__ __launch_bounds__(256) void k_compose(rlevox_raymap rm, const uint32_t* __restrict__ raybuf,
@@ -51,6 +64,16 @@ hipError_t launch_raycast_plain(const SceneDev& sc, const rlevox_raymap& rm, uin
                                 unsigned long long* counts) {
   const int lines = rm.map_line_count;
   if (lines <= 0) return hipSuccess;
+  if (block == 1) {  // one wavefront per line
+    const size_t lds = (size_t)bitmap_words(rm.rs) * sizeof(uint32_t);
+    if (counts)
+      hipLaunchKernelGGL(k_raycast_wave<true>, dim3(lines), dim3(64), lds, s, sc, rm, raybuf, clip,
+                         counts);
+    else
+      hipLaunchKernelGGL(k_raycast_wave<false>, dim3(lines), dim3(64), lds, s, sc, rm, raybuf,
+                         clip, counts);
+    return hipGetLastError();
+  }
   const int grid = (lines + block - 1) / block;
   const size_t lds = (size_t)bitmap_words(rm.rs) * block * sizeof(uint32_t);
   if (counts)

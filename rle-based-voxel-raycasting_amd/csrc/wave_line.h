@@ -1,0 +1,436 @@
+// Wavefront-per-line raycast: the 64 lanes of one wavefront walk one ray
+// line together, producing bit-identical results to render_line().
+//
+//  1. DDA chunk: the line's DDA state (uniform) is advanced up to 64 steps;
+//     lane k keeps step k.  Every fp accumulation runs in the reference's
+//     order (Cuda_Render.h:376-414), so cell sequence and distances are
+//     exactly those of the sequential walk.
+//  2. Gather: each lane loads its cell's pointer-map entry: one round trip
+//     for 64 steps instead of one per step.
+//  3. Stateless projection per lane: clip-top inputs and the projected top
+//     (s2) of the first drawn element in front of the eye (F).
+//  4. Ballot scan: with the current floating horizon ymin, a step whose
+//     clip-top test culls it, or whose F <= ymin, leaves every piece of line
+//     state untouched (the element loop breaks before any update,
+//     Cuda_Render.h:467,543).  A ballot finds the first step that can act;
+//     only those steps run the full element loop, in order.
+//  5. Spans are drawn 64 pixels at a time: lane k accumulates the
+//     perspective terms k times from the span start, exactly as the
+//     per-pixel loop does (Cuda_Render.h:687), and the LDS bitmap of the
+//     line is tested and set per lane.
+#pragma once
+
+#include "device.h"
+
+namespace rlevox {
+
+__device__ inline float readlane_f(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ inline int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+__device__ inline bool lds_bit(const uint32_t* bits, int y) { return (bits[y >> 5] >> (y & 31)) & 1u; }
+
+// First y in [from, to) whose bit is clear, or `to` (from <= to).
+__device__ inline int first_clear(const uint32_t* bits, int from, int to, int lane) {
+  for (int base = from; base < to; base += 64) {
+    const int y = base + lane;
+    const uint64_t m = __ballot(y < to && !lds_bit(bits, y));
+    if (m) return base + __builtin_ctzll(m);
+  }
+  return to;
+}
+
+template <bool COUNT>
+__device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_raymap& rm,
+                                        uint32_t* __restrict__ row, int32_t* __restrict__ clip,
+                                        uint32_t* bits, unsigned long long* counts) {
+  const int lane = threadIdx.x & 63;
+  const int rs = rm.rs;
+  const float half = (float)(rs / 2);
+  if (lane == 0) {
+    clip[0] = 0;
+    clip[1] = -1;
+  }
+
+  // ---- line plane (Cuda_Render.h:128-172), uniform
+  const int r0 = rm.res[0], r1 = r0 + rm.res[1], r2 = r1 + rm.res[2];
+  int q = 0;
+  if (x >= r2) q = 3;
+  else if (x >= r1) q = 2;
+  else if (x >= r0) q = 1;
+  float qofs = (float)x;
+  if (q == 1) qofs -= (float)r0;
+  if (q == 2) qofs -= (float)r1;
+  if (q == 3) qofs -= (float)r2;
+  const float a = qofs / (float)rm.res[q];
+  F3 p1 = f3ld(rm.p_2d[5]);
+  const F3 e0 = f3ld(rm.p_no[2 * q]);
+  F3 p2 = e0 + (f3ld(rm.p_no[2 * q + 1]) - e0) * a;
+  {
+    const float s2 = line_scale(p1, p2, rm.clip_max, rm.clip_min);
+    const F3 c2 = p1 + (p2 - p1) * s2;
+    const float s1 = line_scale(p2, p1, rm.clip_max, rm.clip_min);
+    const F3 c1 = p2 + (p1 - p2) * s1;
+    p1 = c1;
+    p2 = c2;
+  }
+  const F3 p1_3d = mat_apply(rm.to3d, p1 * 4.0f);
+  const F3 p2_3d = mat_apply(rm.to3d, p2 * 4.0f);
+  F3 dir = (p1_3d + p2_3d) * 0.5f - f3ld(rm.p4);
+  dir.y = 0;
+  {
+    const float d = dir.x * dir.x + dir.y * dir.y + dir.z * dir.z;
+    const float inv = 1.0f / sqrtf(d);
+    dir = dir * inv;
+  }
+  const float ray_x = rm.cos_myaw * dir.x + rm.sin_myaw * dir.z;
+  const float ray_z = rm.cos_myaw * dir.z - rm.sin_myaw * dir.x;
+  const bool vertical = q < 2;
+  const float sin_x = rm.sin_x, cos_x = rm.cos_x;
+  bool reverse = false;
+  if (vertical && ray_z <= 0) reverse = true;
+  if (!vertical && ray_x <= 0 && sin_x > 0) reverse = true;
+  if (!vertical && ray_x > 0 && sin_x < 0) reverse = true;
+  float hmul = reverse ? -half : half;
+  if (vertical) hmul = -hmul;
+  const int p_add = reverse ? 1 : -2;
+  int p1x = d_f2i((float)rs * p1.x) + p_add;
+  int p1y = d_f2i((float)rs * p1.y) + p_add;
+  int p2x = d_f2i((float)rs * p2.x) - p_add;
+  int p2y = d_f2i((float)rs * p2.y) - p_add;
+  p1x = min(max(p1x, 0), rs - 1);
+  p1y = min(max(p1y, 0), rs - 1);
+  p2x = min(max(p2x, 0), rs - 1);
+  p2y = min(max(p2y, 0), rs - 1);
+  if (p1y == p2y) return;
+  int ymin, ymax;
+  if (vertical) {
+    ymin = rs - 1 - p1y;
+    ymax = rs - 1 - p2y;
+  } else {
+    ymin = rs - 1 - p1x;
+    ymax = rs - 1 - p2x;
+  }
+  if (reverse) {
+    ymin = rs - 1 - ymin;
+    ymax = rs - 1 - ymax;
+  }
+  if (ymin > ymax) {
+    const int t = ymin;
+    ymin = ymax;
+    ymax = t;
+  }
+  if (ymin >= ymax) return;
+
+  // ---- clear the range and the bitmap (Cuda_Render.h:255-264), coalesced
+  for (int n = ymin + lane; n <= ymax; n += 64) row[n] = RLEVOX_CLEAR_TEXEL;
+  const int words = bitmap_words(rs);
+  for (int w = lane; w < words; w += 64) bits[w] = 0;
+  if (lane == 0) {
+    clip[0] = ymin;
+    clip[1] = ymax;
+  }
+  __syncthreads();
+  LineCounts cnt;
+  if (COUNT) cnt.cleared = (uint64_t)(ymax - ymin + 1);
+
+  // ---- DDA setup (Cuda_Render.h:270-304), uniform
+  const float vpx = rm.position[0], vpy = rm.position[1], vpz = rm.position[2];
+  const float ddx = ray_x * rm.cos_y + ray_z * rm.sin_y;
+  const float ddy = ray_x * rm.sin_y - ray_z * rm.cos_y;
+  float fix_xf = -1, fix_yf = -1, sign_x = -1, sign_y = -1;
+  float frac_x = vpx - (float)d_f2i(vpx);
+  float frac_y = vpz - (float)d_f2i(vpz);
+  if (ddx >= 0) {
+    fix_xf = 0;
+    sign_x = 1;
+    frac_x = 1 - frac_x;
+  }
+  if (ddy >= 0) {
+    fix_yf = 0;
+    sign_y = 1;
+    frac_y = 1 - frac_y;
+  }
+  float g0y = ddy / fabsf(ddx), g0x = sign_x;
+  float g1x = ddx / fabsf(ddy), g1y = sign_y;
+  float i0x = g0x * frac_x, i0y = g0y * frac_x;
+  float i1x = g1x * frac_y, i1y = g1y * frac_y;
+  float gd0 = sqrtf(g0x * g0x + g0y * g0y);
+  float gd1 = sqrtf(g1x * g1x + g1y * g1y);
+  float dd0 = sqrtf(i0x * i0x + i0y * i0y);
+  float dd1 = sqrtf(i1x * i1x + i1y * i1y);
+  float px = 0, py = 0, dnow = 0;
+  int idx = 0;
+
+  int mip = 0;
+  int gridx = sc.mip[0].sx, gridz = sc.mip[0].sz;
+  const uint2* map = sc.mip[0].map;
+  const uint16_t* slab = sc.mip[0].slabs;
+  const float z_add = sin_x;
+  const float y_add = (vertical ? cos_x : 0.0f) * hmul;
+  const int64_t zfar = rm.far_distance;
+  int mapswitch = rm.mapswitch;
+  float dz = 1, z = 0;
+  float yswitch = vpy;
+  const float mountain = vpy;
+
+  for (;;) {
+    // ---- distance mip switch (Cuda_Render.h:343-365), uniform
+    while (z > (float)mapswitch || yswitch > 512.0f) {
+      yswitch = yswitch * 0.5f;
+      if (mip < sc.nmips - 1) {
+        mip++;
+        gridx >>= 1;
+        gridz >>= 1;
+        map = sc.mip[mip].map;
+        slab = sc.mip[mip].slabs;
+      }
+      g0x *= 2;
+      g0y *= 2;
+      g1x *= 2;
+      g1y *= 2;
+      gd0 *= 2;
+      gd1 *= 2;
+      mapswitch *= 2;
+      dz *= 2;
+    }
+    if (ymin >= ymax) break;
+    // Iterations left before the next mip switch and before z passes z_far
+    // (z, dz and mapswitch are exact integers).
+    const int64_t zi = (int64_t)z, dzi = (int64_t)dz;
+    const int64_t n_mip = ((int64_t)mapswitch - zi) / dzi + 1;
+    const int64_t n_far = (zfar - zi) / dzi;
+    if (n_far <= 0) break;
+    const int L = (int)min((int64_t)64, min(n_mip, n_far));
+
+    // ---- 1. DDA chunk: lane k keeps step k
+    float my_db = 0, my_dn = 0, my_pbx = 0, my_pby = 0;
+    int my_ib = 0;
+    for (int k = 0; k < L; k++) {
+      const float db = dnow, pbx = px, pby = py;
+      const int ib = idx;
+      if (dd1 < dd0) {
+        dnow = dd1;
+        idx = 1;
+        dd1 += gd1;
+        px = i1x;
+        py = i1y;
+        i1x += g1x;
+        i1y += g1y;
+      } else {
+        dnow = dd0;
+        idx = 0;
+        px = i0x;
+        py = i0y;
+        dd0 += gd0;
+        i0x += g0x;
+        i0y += g0y;
+      }
+      if (k == lane) {
+        my_db = db;
+        my_dn = dnow;
+        my_pbx = pbx;
+        my_pby = pby;
+        my_ib = ib;
+      }
+    }
+    z += (float)L * dz;
+
+    // ---- 2./3. per-lane cell, gather and stateless projection
+    const bool valid = lane < L;
+    const int fix_x = (int)((float)(1 - my_ib) * fix_xf);
+    const int fix_z = (int)((float)my_ib * fix_yf);
+    const float ddelta = my_dn - my_db;
+    const float vs_x = ray_x * my_db;
+    const float vs_z = ray_z * my_db;
+    const int voxel_x = d_f2i(vpx + my_pbx) + fix_x;
+    const int voxel_z = d_f2i(vpz + my_pby) + fix_z;
+    const int vx = (voxel_x >> mip) & (gridx - 1);
+    const int vz = (voxel_z >> mip) & (gridz - 1);
+    const float corr_x = ray_x * ddelta;
+    const float corr_z = ray_z * ddelta;
+    const float pz = cos_x * vs_z + sin_x * mountain;
+    float py3 = vertical ? cos_x * mountain - sin_x * vs_z : vs_x;
+    py3 *= hmul;
+    const float czz = cos_x * corr_z;
+    float cyy = vertical ? -sin_x * corr_z : corr_x;
+    cyy *= hmul;
+    const float top = pz * half + py3;
+    uint2 ent = make_uint2(0u, 0u);
+    if (valid) ent = map[vx + vz * gridx];
+    const uint32_t slen = ent.y & 0xffffu;
+    // F: projected top of the first drawn element in front of the eye.
+    int F = INT32_MIN;
+    uint32_t nel = 0;
+    {
+      const uint16_t* runs = slab + 2 + ent.x;
+      uint32_t w = ent.y >> 16;
+      int yg = 0;
+      for (uint32_t k = 0; k < slen; k++) {
+        if (k > 0) w = runs[k];
+        nel++;
+        int dr = (int)(w >> 10);
+        yg += (int)(w & 1023u) << mip;
+        if (dr == 0) continue;
+        dr <<= mip;
+        const float ytop = (float)yg;
+        yg += dr;
+        float zz1 = pz, yy1 = py3;
+        if (mountain + ytop >= 0) {
+          zz1 += czz;
+          yy1 += cyy;
+        }
+        const float pz1 = zz1 + z_add * ytop;
+        if (pz1 <= 0) continue;
+        const float py1 = yy1 + y_add * ytop;
+        F = d_f2i(half + py1 / pz1);
+        break;
+      }
+    }
+
+    // ---- 4. ballot scan over the chunk's steps, in order
+    int cur = 0;
+    bool done = false;
+    for (;;) {
+      if (ymin >= ymax) {
+        done = true;
+        break;
+      }
+      const bool culled = top <= pz * (float)ymin;
+      const bool act = valid && lane >= cur && !culled && F > ymin;
+      const uint64_t am = __ballot(act);
+      const int t = am ? (int)__builtin_ctzll(am) : L;
+      if (COUNT) {
+        // no-op steps [cur, t): pointer loads and elements walked to the break
+        const bool noop_loaded = valid && lane >= cur && lane < t && !culled;
+        cnt.steps += (uint64_t)(t - cur);
+        cnt.ptr += (uint64_t)__popcll(__ballot(noop_loaded));
+        unsigned long long e = noop_loaded ? nel : 0, r = (noop_loaded && nel) ? nel - 1 : 0;
+        for (int o = 32; o > 0; o >>= 1) {
+          e += __shfl_xor(e, o);
+          r += __shfl_xor(r, o);
+        }
+        cnt.elems += e;
+        cnt.rle += r;
+      }
+      if (!am) break;
+
+      // ---- full element loop of step t (Cuda_Render.h:474-734), uniform
+      if (COUNT) {
+        cnt.steps++;
+        cnt.ptr++;
+      }
+      const float tpz = readlane_f(pz, t), tpy = readlane_f(py3, t);
+      const float tczz = readlane_f(czz, t), tcyy = readlane_f(cyy, t);
+      const uint32_t tex_ofs = (uint32_t)readlane_i((int)ent.x, t);
+      const uint32_t tlf = (uint32_t)readlane_i((int)ent.y, t);
+      const uint32_t tslen = tlf & 0xffffu;
+      const uint16_t* runs = slab + 2 + tex_ofs;
+      const uint16_t* colours = runs + tslen;
+      uint32_t w = tlf >> 16;
+      int ygen = 0;
+      float tex = 0;
+      for (uint32_t k = 0; k < tslen; k++) {
+        if (k > 0) w = runs[k];
+        if (COUNT) {
+          cnt.elems++;
+          if (k > 0) cnt.rle++;
+        }
+        int drawn = (int)(w >> 10);
+        ygen += (int)(w & 1023u) << mip;
+        if (drawn == 0) continue;
+        const int texture = (int)tex;
+        tex += (float)drawn;
+        drawn <<= mip;
+        const float ytop = (float)ygen;
+        ygen += drawn;
+        const float ybot = (float)ygen;
+        float zz1 = tpz, yy1 = tpy;
+        if (mountain + ytop >= 0) {
+          zz1 += tczz;
+          yy1 += tcyy;
+        }
+        const float pz1 = zz1 + z_add * ytop;
+        if (pz1 <= 0) continue;
+        const float py1 = yy1 + y_add * ytop;
+        int s2 = d_f2i(half + py1 / pz1);
+        if (s2 <= ymin) break;
+        float zz2 = tpz, yy2 = tpy;
+        if (mountain + ybot < 0) {
+          zz2 += tczz;
+          yy2 += tcyy;
+        }
+        const float pz2 = zz2 + z_add * ybot;
+        if (pz2 <= 0) continue;
+        const float py2 = yy2 + y_add * ybot;
+        int s1 = d_f2i(half + py2 / pz2 - 1.0f);
+        if (s1 >= ymax) continue;
+        if (s2 >= ymax) {
+          s2 = ymax;
+          ymax = s1;
+        }
+        if (s1 <= ymin) {
+          s1 = ymin;
+          ymin = s2;
+          if (ymax > ymin) ymin = first_clear(bits, ymin, ymax, lane);
+        }
+        const int y0 = first_clear(bits, s1, s2, lane);
+        if (y0 >= s2) continue;
+        const float pz1r = tpz + z_add * ytop;
+        const float py1r = tpy + y_add * ytop;
+        const float pz2r = tpz + z_add * ybot;
+        const float py2r = tpy + y_add * ybot;
+        const float s2r = half + py1r / pz1r;
+        const float s1r = half + py2r / pz2r;
+        const float u1z = tex / pz2r;
+        float u2dz = (float)texture / pz1r - u1z;
+        const float onez1 = 1.0f / pz2r;
+        float onedz2 = 1.0f / pz1r - onez1;
+        u2dz /= s2r - s1r;
+        onedz2 /= s2r - s1r;
+        const float mult = (float)(y0 + 1) - s1r;
+        float uzb = u1z + u2dz * mult;
+        float ozb = onez1 + onedz2 * mult;
+        if (COUNT) cnt.spans++;
+        const float tlo = (float)texture, thi = tex - 1.0f;
+        // ---- 5. span, 64 pixels per pass
+        for (int base = y0; base < s2; base += 64) {
+          float uz = uzb, oz = ozb;
+          for (int i = 0; i < lane; i++) {
+            uz += u2dz;
+            oz += onedz2;
+          }
+          const int y = base + lane;
+          const bool draw = y < s2 && !lds_bit(bits, y);
+          if (draw) {
+            const float fu = fminf(fmaxf(uz / oz, tlo), thi);
+            const uint32_t u = (uint32_t)d_f2i(fu);
+            const uint32_t depth = (uint32_t)d_f2i(1.0f / oz) & 0xfffeu;
+            row[y] = (uint32_t)colours[u] + (depth << 16);
+          }
+          if (COUNT) cnt.col += (uint64_t)__popcll(__ballot(draw));
+          __syncthreads();
+          if (draw) atomicOr(&bits[y >> 5], 1u << (y & 31));
+          __syncthreads();
+          uzb = readlane_f(uz, 63) + u2dz;
+          ozb = readlane_f(oz, 63) + onedz2;
+        }
+      }
+      cur = t + 1;
+    }
+    if (done) break;
+  }
+  if (COUNT && lane == 0) {
+    atomicAdd(counts + 0, 1ull);
+    atomicAdd(counts + 1, (unsigned long long)cnt.steps);
+    atomicAdd(counts + 2, (unsigned long long)cnt.ptr);
+    atomicAdd(counts + 3, (unsigned long long)cnt.rle);
+    atomicAdd(counts + 4, (unsigned long long)cnt.elems);
+    atomicAdd(counts + 5, (unsigned long long)cnt.spans);
+    atomicAdd(counts + 6, (unsigned long long)cnt.col);
+    atomicAdd(counts + 7, (unsigned long long)cnt.cleared);
+  }
+}
+
+}  // namespace rlevox

@@ -382,7 +382,7 @@ class Context:
         check(lib.rlevox_event_elapsed_ms(self.handle, a, b, C.byref(ms)))
         return float(ms.value)
 
-    def set_tuning(self, lines_per_wave=64, waves_per_block=1, prefetch=0, order=1):
+    def set_tuning(self, lines_per_wave=1, waves_per_block=1, prefetch=0, order=1):
         t = Tuning(lines_per_wave, waves_per_block, prefetch, order)
         check(lib.rlevox_set_tuning(self.handle, C.byref(t)))
 

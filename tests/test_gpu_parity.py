@@ -50,6 +50,11 @@ def assert_same(name, got, want):
                              % (name, len(diff), got.size, i, got[i], want[i]))
 
 
+# Raycast kernel variants (lines per wavefront): one lane per line in blocks
+# of 64 or 16 lanes, and one wavefront per line.  All must agree bit-exactly.
+KERNELS = [(64, 1), (16, 1), (1, 1)]
+
+
 def check_view(ctx, oracle, dscene, mips, pose, W, H, mip_distance=0, far_distance=0):
     rm = rlevox.raymap(pose, W, H, mip_distance, far_distance)
     orm = oracle.raymap(pose, W, H, mip_distance, far_distance)
@@ -57,13 +62,17 @@ def check_view(ctx, oracle, dscene, mips, pose, W, H, mip_distance=0, far_distan
     ref_rb, ref_clip, _ = oracle.raycast(mips, orm)
     ref_col, ref_dep = oracle.compose(orm, ref_rb, ref_clip)
     ref_fin = oracle.gapfill(ref_col, ref_dep)
-    got = gpu_stages(ctx, dscene, rm)
     n = max(rm.map_line_count, 1)
-    assert_same("line clip", got["clip"][:n], ref_clip[:n])
-    assert_same("ray buffer", got["raybuf"], ref_rb)
-    assert_same("compose colour", got["compose"], ref_col)
-    assert_same("depth", got["depth"], ref_dep)
-    assert_same("gap-filled colour", got["colour"], ref_fin)
+    for lpw, wpb in KERNELS:
+        ctx.set_tuning(lines_per_wave=lpw, waves_per_block=wpb)
+        got = gpu_stages(ctx, dscene, rm)
+        tag = " [lines/wave %d]" % lpw
+        assert_same("line clip" + tag, got["clip"][:n], ref_clip[:n])
+        assert_same("ray buffer" + tag, got["raybuf"], ref_rb)
+        assert_same("compose colour" + tag, got["compose"], ref_col)
+        assert_same("depth" + tag, got["depth"], ref_dep)
+        assert_same("gap-filled colour" + tag, got["colour"], ref_fin)
+    ctx.set_tuning()
     return rm
 
 
@@ -142,8 +151,11 @@ def test_work_counters_match_oracle(ctx, oracle, c1_scene, c1_dev):
         lines = max(rm.map_line_count, 1)
         d_rb = ctx.alloc(lines * rm.rs * 4)
         d_clip = ctx.alloc(lines * 8)
-        got = ctx.raycast_counters(c1_dev, rm, d_rb, d_clip)
         _, _, want = oracle.raycast(c1_scene.mips, oracle.raymap(p, 1280, 720))
-        assert got == want
+        for lpw, wpb in KERNELS:
+            ctx.set_tuning(lines_per_wave=lpw, waves_per_block=wpb)
+            got = ctx.raycast_counters(c1_dev, rm, d_rb, d_clip)
+            assert got == want, (lpw, got, want)
+        ctx.set_tuning()
         d_rb.free()
         d_clip.free()
