@@ -148,7 +148,7 @@ def main():
                    "post_ms_mean": round(float(np.mean(post_ms)), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None, "kernel": "k_raycast_plain",
+                     "traffic": None, "kernel": "k_raycast_wave",
                      "bytes_per_launch": round(mean_bytes)},
     }
 

@@ -111,7 +111,8 @@ rlevox_status rlevox_scene_upload(rlevox_ctx* ctx, const rlevox_mip* mips, int32
   s->dev.nmips = nmips;
   for (int i = 0; i < nmips; i++) {
     const size_t map_bytes = (size_t)8 * m[i].sx * m[i].sz;
-    const size_t slab_bytes = ((size_t)m[i].nslabs * 2 + 255) & ~(size_t)255;
+    // padded so whole-cache-line reads that start inside the array stay in it
+    const size_t slab_bytes = (((size_t)m[i].nslabs * 2 + 255) & ~(size_t)255) + 512;
     void* dm = nullptr;
     void* ds = nullptr;
     hipError_t e = hipMalloc(&dm, map_bytes);
@@ -131,6 +132,17 @@ rlevox_status rlevox_scene_upload(rlevox_ctx* ctx, const rlevox_mip* mips, int32
     s->sy[i] = m[i].sy;
     s->bytes += map_bytes + slab_bytes;
   }
+  void* dd = nullptr;
+  hipError_t e = hipMalloc(&dd, sizeof(SceneDev));
+  if (e == hipSuccess) {
+    s->allocs.push_back(dd);
+    e = hipMemcpy(dd, &s->dev, sizeof(SceneDev), hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    rlevox_scene_free(s);
+    return hip_fail(e, "scene_upload");
+  }
+  s->d_dev = (SceneDev*)dd;
   *out = s;
   return RLEVOX_OK;
 }

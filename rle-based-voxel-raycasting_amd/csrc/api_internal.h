@@ -62,6 +62,28 @@ inline rlevox_status check_raymap(const rlevox_raymap& rm) {
 
 rlevox_status ctx_bind(rlevox_ctx* ctx);
 
+// Device state of one in-flight batch: ray buffers, descriptors, pinned
+// staging and the captured graph (replayed for any batch: the kernels read
+// the batch shape from the descriptor block, whose address never changes).
+struct BatchSlot {
+  static constexpr int kMaxViews = 4096;
+  static constexpr int kWavesPerCU = 16;
+  rlevox_ctx* ctx = nullptr;
+  hipStream_t stream = nullptr;
+  Scratch scratch, descs;
+  void* staging = nullptr;
+  size_t staging_size = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int nviews = 0;
+  uint64_t rays = 0;
+  rlevox_status init(rlevox_ctx* c);
+  rlevox_status prepare(const rlevox_view* views, int n, const rlevox_render_params* params,
+                        const rlevox_tuning& tuning);
+  rlevox_status run(bool use_graph, hipStream_t s);
+  void release();
+};
+
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t count, hipStream_t s);
 
 }  // namespace rlevox
@@ -72,6 +94,7 @@ struct rlevox_ctx {
   char arch[64] = {0};
   std::mutex mu;  // guards the scratch buffers and tuning
   rlevox::Scratch raybuf, clip, ctmp, counts;
+  rlevox::BatchSlot batch;
   rlevox_tuning tuning{1, 1, 0, 1};
   // 1 selects the wavefront-per-line kernel; otherwise lanes per block.
   int tuning_block() const {
@@ -82,6 +105,7 @@ struct rlevox_ctx {
     clip.release();
     ctmp.release();
     counts.release();
+    batch.release();
   }
 };
 
@@ -90,6 +114,7 @@ struct rlevox_scene {
   int nmips = 0;
   int sy[RLEVOX_MAX_MIPS] = {0};
   rlevox::SceneDev dev;
+  rlevox::SceneDev* d_dev = nullptr;  // device copy, read by batched kernels
   std::vector<void*> allocs;
   uint64_t bytes = 0;
 };

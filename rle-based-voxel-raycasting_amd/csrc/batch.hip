@@ -1,0 +1,284 @@
+// Batched rendering of many views in three launches: a persistent raycast
+// whose wavefronts pull (view, line) items from a device work queue, then
+// compose and gap-fill over the pixel tiles of every view.  Each view's
+// output is bit-identical to rlevox_render's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "api_internal.h"
+#include "batch.h"
+#include "kernels.h"
+#include "wave_line.h"
+
+namespace rlevox {
+
+__global__ void k_batch_reset(BatchHeader* h) {
+  if (threadIdx.x == 0) h->queue = 0;
+}
+
+// Largest i with begin(i) <= g (views are ordered by begin).
+template <bool TILES>
+__device__ inline int find_view(const ViewDesc* __restrict__ v, int n, int64_t g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int64_t b = TILES ? v[mid].tile_begin : v[mid].line_begin;
+    if (b <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Persistent: each wavefront takes the next line of the batch until the
+// queue is drained; every wavefront reaches the exit test each iteration.
+__global__ __launch_bounds__(64) void k_raycast_batch(BatchHeader* __restrict__ h,
+                                                      const ViewDesc* __restrict__ views) {
+  __shared__ uint32_t bits[(RLEVOX_MAX_RS + 31) / 32];
+  const int lane = threadIdx.x;
+  const int64_t total = h->total_lines;
+  const int nv = h->nviews;
+  for (;;) {
+    uint32_t g = 0;
+    if (lane == 0) g = atomicAdd(&h->queue, 1u);
+    g = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)g, 0));
+    if ((int64_t)g >= total) break;
+    const int v = find_view<false>(views, nv, (int64_t)g);
+    const ViewDesc& vd = views[v];
+    const int x = (int)((int64_t)g - vd.line_begin);
+    render_line_wave<false>(x, *vd.scene, vd.rm, vd.raybuf + (size_t)x * (size_t)vd.rm.rs,
+                            vd.clip + 2 * x, bits, nullptr);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_compose_batch(const BatchHeader* __restrict__ h,
+                                                       const ViewDesc* __restrict__ views) {
+  const int64_t total = h->total_tiles;
+  const int nv = h->nviews;
+  for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    const int v = find_view<true>(views, nv, tile);
+    const ViewDesc& vd = views[v];
+    const int W = vd.rm.W, H = vd.rm.H;
+    const int64_t local = tile - vd.tile_begin;
+    const int tiles_x = (W + 63) / 64;
+    const int px = (int)(local % tiles_x) * 64 + (threadIdx.x & 63);
+    const int row = (int)(local / tiles_x) * 4 + (threadIdx.x >> 6);
+    if (px < W && row < H) {
+      uint16_t c;
+      int32_t d;
+      compose_pixel(vd.rm, vd.raybuf, vd.clip, px, row, c, d);
+      uint16_t* out = (vd.post & RLEVOX_POST_GAPFILL) ? vd.ctmp : vd.colour;
+      out[(size_t)row * W + px] = c;
+      vd.depth[(size_t)row * W + px] = d;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gapfill_batch(const BatchHeader* __restrict__ h,
+                                                       const ViewDesc* __restrict__ views) {
+  const int64_t total = h->total_tiles;
+  const int nv = h->nviews;
+  for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    const int v = find_view<true>(views, nv, tile);
+    const ViewDesc& vd = views[v];
+    if (!(vd.post & RLEVOX_POST_GAPFILL)) continue;
+    const int W = vd.rm.W, H = vd.rm.H;
+    const int64_t local = tile - vd.tile_begin;
+    const int tiles_x = (W + 63) / 64;
+    const int px = (int)(local % tiles_x) * 64 + (threadIdx.x & 63);
+    const int row = (int)(local / tiles_x) * 4 + (threadIdx.x >> 6);
+    if (px < W && row < H) vd.colour[(size_t)row * W + px] = gapfill_pixel(W, H, vd.ctmp, vd.depth, px, row);
+  }
+}
+
+hipError_t launch_batch(BatchHeader* d_hdr, const ViewDesc* d_views, int raycast_grid, int post_grid,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, d_hdr);
+  hipLaunchKernelGGL(k_raycast_batch, dim3(raycast_grid), dim3(64), 0, s, d_hdr, d_views);
+  hipLaunchKernelGGL(k_compose_batch, dim3(post_grid), dim3(256), 0, s, d_hdr, d_views);
+  hipLaunchKernelGGL(k_gapfill_batch, dim3(post_grid), dim3(256), 0, s, d_hdr, d_views);
+  return hipGetLastError();
+}
+
+static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+rlevox_status BatchSlot::init(rlevox_ctx* c) {
+  ctx = c;
+  if (!stream) {
+    hipError_t e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "batch stream");
+  }
+  return RLEVOX_OK;
+}
+
+void BatchSlot::release() {
+  if (exec) hipGraphExecDestroy(exec);
+  if (graph) hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+  scratch.release();
+  descs.release();
+  if (staging) hipHostFree(staging);
+  staging = nullptr;
+  staging_size = 0;
+  if (stream) hipStreamDestroy(stream);
+  stream = nullptr;
+}
+
+rlevox_status BatchSlot::prepare(const rlevox_view* views, int n, const rlevox_render_params* params,
+                                 const rlevox_tuning& tuning) {
+  if (!views || n < 1 || n > kMaxViews)
+    return fail(RLEVOX_ERR_INVALID_ARG, "render_batch: need 1..%d views", kMaxViews);
+  const int post = post_flags(params);
+  std::vector<rlevox_raymap> rms(n);
+  std::vector<double> cost(n);
+  for (int i = 0; i < n; i++) {
+    const rlevox_view& v = views[i];
+    if (!v.scene || !v.d_colour || !v.d_depth)
+      return fail(RLEVOX_ERR_INVALID_ARG, "render_batch: view %d has a null pointer", i);
+    if (v.scene->ctx != ctx)
+      return fail(RLEVOX_ERR_INVALID_ARG, "render_batch: view %d scene belongs to another context", i);
+    rlevox_status st = raymap_compute(v.pose, v.W, v.H, params ? params->mip_distance : 0,
+                                      params ? params->far_distance : 0, rms[i]);
+    if (st == RLEVOX_OK) st = check_raymap(rms[i]);
+    if (st != RLEVOX_OK) return st;
+    cost[i] = (double)rms[i].map_line_count * rms[i].rs;
+  }
+  std::vector<int> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  if (tuning.order == 1)
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  // scratch layout: per view ray buffer, line clip ranges, composed colour
+  size_t bytes = 0;
+  std::vector<size_t> off_rb(n), off_clip(n), off_ct(n);
+  for (int k = 0; k < n; k++) {
+    const int i = order[k];
+    off_rb[k] = bytes;
+    bytes += align256(raybuf_bytes(rms[i]));
+    off_clip[k] = bytes;
+    bytes += align256(clip_bytes(rms[i]));
+    off_ct[k] = bytes;
+    bytes += (post & RLEVOX_POST_GAPFILL) ? align256((size_t)rms[i].W * rms[i].H * 2) : 0;
+  }
+  rlevox_status st = scratch.ensure(bytes ? bytes : 256);
+  if (st != RLEVOX_OK) return st;
+  const size_t dbytes = align256(sizeof(BatchHeader)) + sizeof(ViewDesc) * (size_t)kMaxViews;
+  if (descs.size < dbytes) {
+    // the descriptor block never moves after this, so a captured graph stays valid
+    if ((st = descs.ensure(dbytes)) != RLEVOX_OK) return st;
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  if (staging_size < dbytes) {
+    if (staging) hipHostFree(staging);
+    staging = nullptr;
+    hipError_t e = hipHostMalloc(&staging, dbytes, hipHostMallocDefault);
+    if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(batch staging)");
+    staging_size = dbytes;
+  }
+  BatchHeader* hh = (BatchHeader*)staging;
+  ViewDesc* hv = (ViewDesc*)((char*)staging + align256(sizeof(BatchHeader)));
+  char* base = (char*)scratch.ptr;
+  int64_t lines = 0, tiles = 0;
+  int max_rs = 16;
+  rays = 0;
+  for (int k = 0; k < n; k++) {
+    const int i = order[k];
+    ViewDesc& d = hv[k];
+    std::memset(&d, 0, sizeof(d));
+    d.rm = rms[i];
+    d.scene = views[i].scene->d_dev;
+    d.raybuf = (uint32_t*)(base + off_rb[k]);
+    d.clip = (int32_t*)(base + off_clip[k]);
+    d.ctmp = (post & RLEVOX_POST_GAPFILL) ? (uint16_t*)(base + off_ct[k]) : nullptr;
+    d.colour = views[i].d_colour;
+    d.depth = views[i].d_depth;
+    d.line_begin = lines;
+    d.tile_begin = tiles;
+    d.post = post;
+    lines += rms[i].map_line_count;
+    tiles += (int64_t)((rms[i].W + 63) / 64) * ((rms[i].H + 3) / 4);
+    max_rs = std::max(max_rs, rms[i].rs);
+    rays += (uint64_t)rms[i].W * rms[i].H;
+  }
+  if (lines > 0xffffffffll) return fail(RLEVOX_ERR_INVALID_ARG, "render_batch: too many lines");
+  std::memset(hh, 0, sizeof(*hh));
+  hh->total_lines = lines;
+  hh->total_tiles = tiles;
+  hh->nviews = n;
+  hh->max_words = bitmap_words(max_rs);
+  nviews = n;
+  return RLEVOX_OK;
+}
+
+rlevox_status BatchSlot::run(bool use_graph, hipStream_t s) {
+  BatchHeader* d_hdr = (BatchHeader*)descs.ptr;
+  ViewDesc* d_views = (ViewDesc*)((char*)descs.ptr + align256(sizeof(BatchHeader)));
+  const size_t copy = align256(sizeof(BatchHeader)) + sizeof(ViewDesc) * (size_t)nviews;
+  hipError_t e = hipMemcpyAsync(descs.ptr, staging, copy, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return hip_fail(e, "batch descriptors");
+  const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
+  const int ray_grid = cus * kWavesPerCU;
+  const int post_grid = cus * 8;
+  if (!use_graph) {
+    e = launch_batch(d_hdr, d_views, ray_grid, post_grid, s);
+    return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "batch launch");
+  }
+  if (!exec) {
+    hipStream_t cap = nullptr;
+    e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      hipError_t le = launch_batch(d_hdr, d_views, ray_grid, post_grid, cap);
+      e = hipStreamEndCapture(cap, &graph);
+      if (e == hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (cap) hipStreamDestroy(cap);
+    if (e != hipSuccess) {
+      if (graph) hipGraphDestroy(graph);
+      graph = nullptr;
+      exec = nullptr;
+      return hip_fail(e, "batch graph capture");
+    }
+  }
+  e = hipGraphLaunch(exec, s);
+  return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "batch graph launch");
+}
+
+static rlevox_status render_batch_impl(rlevox_ctx* ctx, const rlevox_view* views, int32_t nviews,
+                                       const rlevox_render_params* params, void* stream,
+                                       bool graph) {
+  clear_error();
+  rlevox_status st = ctx_bind(ctx);
+  if (st != RLEVOX_OK) return st;
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  if ((st = ctx->batch.init(ctx)) != RLEVOX_OK) return st;
+  if ((st = ctx->batch.prepare(views, nviews, params, ctx->tuning)) != RLEVOX_OK) return st;
+  hipStream_t s = (hipStream_t)stream;
+  if ((st = ctx->batch.run(graph, s)) != RLEVOX_OK) return st;
+  // the slot's scratch and staging are reused by the next call
+  hipError_t e = hipStreamSynchronize(s);
+  return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "render_batch");
+}
+
+}  // namespace rlevox
+
+using namespace rlevox;
+
+extern "C" rlevox_status rlevox_render_batch(rlevox_ctx* ctx, const rlevox_view* views,
+                                             int32_t nviews, const rlevox_render_params* params,
+                                             void* stream) {
+  return render_batch_impl(ctx, views, nviews, params, stream, false);
+}
+
+extern "C" rlevox_status rlevox_render_batch_graph(rlevox_ctx* ctx, const rlevox_view* views,
+                                                   int32_t nviews, const rlevox_render_params* params,
+                                                   void* stream) {
+  return render_batch_impl(ctx, views, nviews, params, stream, true);
+}

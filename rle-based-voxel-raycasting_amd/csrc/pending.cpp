@@ -4,12 +4,6 @@
 using namespace rlevox;
 
 extern "C" {
-rlevox_status rlevox_render_batch(rlevox_ctx*, const rlevox_view*, int32_t, const rlevox_render_params*, void*) {
-  return fail(RLEVOX_ERR_INVALID_ARG, "render_batch: not available in this build");
-}
-rlevox_status rlevox_render_batch_graph(rlevox_ctx*, const rlevox_view*, int32_t, const rlevox_render_params*, void*) {
-  return fail(RLEVOX_ERR_INVALID_ARG, "render_batch_graph: not available in this build");
-}
 rlevox_status rlevox_autotune(rlevox_ctx*, const rlevox_view*, int32_t, const rlevox_render_params*, rlevox_tuning*, double*) {
   return fail(RLEVOX_ERR_INVALID_ARG, "autotune: not available in this build");
 }

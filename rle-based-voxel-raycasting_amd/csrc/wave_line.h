@@ -397,7 +397,10 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
         // ---- 5. span, 64 pixels per pass
         for (int base = y0; base < s2; base += 64) {
           float uz = uzb, oz = ozb;
-          for (int i = 0; i < lane; i++) {
+          // pixel base+lane has had `lane` additions; lanes past the span
+          // end stop early (their values are not used)
+          const int adds = min(lane, s2 - 1 - base);
+          for (int i = 0; i < adds; i++) {
             uz += u2dz;
             oz += onedz2;
           }
