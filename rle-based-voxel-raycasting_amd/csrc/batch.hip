@@ -38,6 +38,8 @@ __device__ inline int find_view(const ViewDesc* __restrict__ v, int n, int64_t g
 __global__ __launch_bounds__(64) void k_raycast_batch(BatchHeader* __restrict__ h,
                                                       const ViewDesc* __restrict__ views) {
   __shared__ uint32_t bits[(RLEVOX_MAX_RS + 31) / 32];
+  __shared__ float dda[kDdaLds];
+  __shared__ uint4 rcache[64 * kRunCache / 8];
   const int lane = threadIdx.x;
   const int64_t total = h->total_lines;
   const int nv = h->nviews;
@@ -50,7 +52,8 @@ __global__ __launch_bounds__(64) void k_raycast_batch(BatchHeader* __restrict__ 
     const ViewDesc& vd = views[v];
     const int x = (int)((int64_t)g - vd.line_begin);
     render_line_wave<false>(x, *vd.scene, vd.rm, vd.raybuf + (size_t)x * (size_t)vd.rm.rs,
-                            vd.clip + 2 * x, bits, nullptr);
+                            vd.clip + 2 * x, bits, dda, reinterpret_cast<uint16_t*>(rcache),
+                            nullptr);
   }
 }
 

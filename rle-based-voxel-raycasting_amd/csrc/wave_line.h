@@ -1,26 +1,36 @@
 // Wavefront-per-line raycast: the 64 lanes of one wavefront walk one ray
-// line together, producing bit-identical results to render_line().
+// line together and produce results bit-identical to render_line().
 //
-//  1. DDA chunk: the line's DDA state (uniform) is advanced up to 64 steps;
-//     lane k keeps step k.  Every fp accumulation runs in the reference's
-//     order (Cuda_Render.h:376-414), so cell sequence and distances are
-//     exactly those of the sequential walk.
-//  2. Gather: each lane loads its cell's pointer-map entry: one round trip
-//     for 64 steps instead of one per step.
-//  3. Stateless projection per lane: clip-top inputs and the projected top
-//     (s2) of the first drawn element in front of the eye (F).
-//  4. Ballot scan: with the current floating horizon ymin, a step whose
-//     clip-top test culls it, or whose F <= ymin, leaves every piece of line
-//     state untouched (the element loop breaks before any update,
-//     Cuda_Render.h:467,543).  A ballot finds the first step that can act;
-//     only those steps run the full element loop, in order.
-//  5. Spans are drawn 64 pixels at a time: lane k accumulates the
-//     perspective terms k times from the span start, exactly as the
-//     per-pixel loop does (Cuda_Render.h:687), and the LDS bitmap of the
-//     line is tested and set per lane.
+//  1. DDA chunk: up to 64 DDA steps per pass, lane k taking step k.  The
+//     crossing distances and positions are repeated fp32 additions
+//     (Cuda_Render.h:398-414); within one binade such a sum advances by a
+//     fixed number of ulps, so each lane computes its term from a few
+//     chunk-wide integers (SeqChunk), and the x/z choice order is the merge
+//     order of the two crossing-distance sequences (ties go to x, because
+//     the walk takes z only when it is strictly closer).  Chunks where a
+//     sequence could cross two binades fall back to the plain loop.
+//  2. Gather: each lane loads its cell's pointer-map entry, one round trip
+//     for the whole chunk.
+//  3. Stateless projection per lane: clip-top inputs and F, the projected
+//     top of the first drawn element in front of the eye.
+//  4. Run cache: each step that may act copies the head of its column
+//     (header, runs, colours) into LDS; it then precomputes, in parallel,
+//     the span parameters of that first element and F2, the projected top
+//     of the next element the loop would test.
+//  5. Ordered replay: with the current floating horizon ymin, a step whose
+//     clip-top test culls it or whose F <= ymin changes no line state (the
+//     element loop breaks first, Cuda_Render.h:467,543).  A ballot finds the
+//     next step that acts; only those steps are replayed, in order, using
+//     the precomputed values, and the loop over further elements runs only
+//     when F2 > ymin.
+//  6. Spans are drawn 64 pixels per pass: lane k applies the per-pixel
+//     increments k times from the span start, exactly as the pixel loop
+//     does (Cuda_Render.h:687), and the line's LDS bitmap is updated from a
+//     ballot of the drawn pixels.
 #pragma once
 
 #include "device.h"
+#include "ffadd.h"
 
 namespace rlevox {
 
@@ -30,6 +40,12 @@ __device__ inline float readlane_f(float v, int l) {
 __device__ inline int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
 __device__ inline bool lds_bit(const uint32_t* bits, int y) { return (bits[y >> 5] >> (y & 31)) & 1u; }
+
+// LDS floats per wavefront used by the DDA chunk.
+constexpr int kDdaLds = 7 * 64;
+// Slab words cached per step (from the 16-byte boundary at or below the
+// column header): covers header, elements and colours of typical columns.
+constexpr int kRunCache = 40;
 
 // First y in [from, to) whose bit is clear, or `to` (from <= to).
 __device__ inline int first_clear(const uint32_t* bits, int from, int to, int lane) {
@@ -41,10 +57,69 @@ __device__ inline int first_clear(const uint32_t* bits, int from, int to, int la
   return to;
 }
 
+// Number of entries of the increasing 64-array `arr` that precede value v:
+// (LE ? arr[l] <= v : arr[l] < v).  `guess` is an estimate of the answer.
+template <bool LE>
+__device__ inline int rank_in(const float* arr, float v, int guess) {
+  int j = min(max(guess, 0), 64);
+  auto before = [&](int l) { return LE ? !(v < arr[l]) : (arr[l] < v); };
+  while (j > 0 && !before(j - 1)) j--;
+  while (j < 64 && before(j)) j++;
+  return j;
+}
+
+// ------------------------------------------------------------ span pixels
+
+// Draws pixels [y0, s2) of one span (uniform arguments; Cuda_Render.h:
+// 678-733).  Pixel y0+k has had k per-pixel increments.
+__device__ inline int draw_span(int y0, int s2, float uzb, float ozb, float u2dz, float onedz2,
+                                float tlo, float thi, const uint16_t* cached, int ccap,
+                                const uint16_t* colours, int cofs, uint32_t* __restrict__ row,
+                                uint32_t* bits, int lane) {
+  int drawn = 0;
+  for (int base = y0; base < s2; base += 64) {
+    float uz = uzb, oz = ozb;
+    const int adds = min(lane, s2 - 1 - base);
+    for (int i = 0; i < adds; i++) {
+      uz += u2dz;
+      oz += onedz2;
+    }
+    const int y = base + lane;
+    const bool draw = y < s2 && !lds_bit(bits, y);
+    if (draw) {
+      const float fu = fminf(fmaxf(uz / oz, tlo), thi);
+      const uint32_t u = (uint32_t)d_f2i(fu);
+      const uint32_t depth = (uint32_t)d_f2i(1.0f / oz) & 0xfffeu;
+      const int cw = cofs + (int)u;
+      const uint32_t col = cw < ccap ? cached[cw] : colours[u];
+      row[y] = col + (depth << 16);
+    }
+    const uint64_t dm = __ballot(draw);
+    drawn += (int)__popcll(dm);
+    if (dm && lane == 0) {
+      // pixel base+i is bit i of dm; spread over at most three words
+      const int w0 = base >> 5, off = base & 31;
+      const uint64_t lo = dm << off;
+      bits[w0] |= (uint32_t)lo;
+      const uint32_t mid = off ? (uint32_t)(dm >> (32 - off)) : (uint32_t)(dm >> 32);
+      if (mid) bits[w0 + 1] |= mid;
+      const uint32_t hi = off ? (uint32_t)(dm >> (64 - off)) : 0u;
+      if (hi) bits[w0 + 2] |= hi;
+    }
+    __syncthreads();
+    if (base + 64 < s2) {
+      uzb = readlane_f(uz, 63) + u2dz;
+      ozb = readlane_f(oz, 63) + onedz2;
+    }
+  }
+  return drawn;
+}
+
 template <bool COUNT>
 __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_raymap& rm,
                                         uint32_t* __restrict__ row, int32_t* __restrict__ clip,
-                                        uint32_t* bits, unsigned long long* counts) {
+                                        uint32_t* bits, float* dl, uint16_t* rc,
+                                        unsigned long long* counts) {
   const int lane = threadIdx.x & 63;
   const int rs = rm.rs;
   const float half = (float)(rs / 2);
@@ -207,32 +282,92 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     // ---- 1. DDA chunk: lane k keeps step k
     float my_db = 0, my_dn = 0, my_pbx = 0, my_pby = 0;
     int my_ib = 0;
-    for (int k = 0; k < L; k++) {
-      const float db = dnow, pbx = px, pby = py;
-      const int ib = idx;
-      if (dd1 < dd0) {
-        dnow = dd1;
-        idx = 1;
-        dd1 += gd1;
-        px = i1x;
-        py = i1y;
-        i1x += g1x;
-        i1y += g1y;
-      } else {
-        dnow = dd0;
-        idx = 0;
-        px = i0x;
-        py = i0y;
-        dd0 += gd0;
-        i0x += g0x;
-        i0y += g0y;
+    const SeqChunk cA = seq_chunk(dd0, gd0), cB = seq_chunk(dd1, gd1);
+    const SeqChunk cX0 = seq_chunk(i0x, g0x), cY0 = seq_chunk(i0y, g0y);
+    const SeqChunk cX1 = seq_chunk(i1x, g1x), cY1 = seq_chunk(i1y, g1y);
+    if (cA.ok && cB.ok && cX0.ok && cY0.ok && cX1.ok && cY1.ok) {
+      float* sA = dl;
+      float* sB = dl + 64;
+      float* sX0 = dl + 128;
+      float* sY0 = dl + 192;
+      float* sX1 = dl + 256;
+      float* sY1 = dl + 320;
+      int* slot = reinterpret_cast<int*>(dl + 384);
+      const float ak = seq_term(cA, lane), bk = seq_term(cB, lane);
+      sA[lane] = ak;
+      sB[lane] = bk;
+      sX0[lane] = seq_term(cX0, lane);
+      sY0[lane] = seq_term(cY0, lane);
+      sX1[lane] = seq_term(cX1, lane);
+      sY1[lane] = seq_term(cY1, lane);
+      __syncthreads();
+      // merge ranks: the sequences grow by about gd per term
+      const int gx = d_f2i(ceilf(__fdividef(ak - dd1, gd1)));
+      const int gz = d_f2i(ceilf(__fdividef(bk - dd0, gd0)));
+      const int rx = lane + rank_in<false>(sB, ak, gx);
+      const int rz = lane + rank_in<true>(sA, bk, gz);
+      if (rx < L) slot[rx] = lane;
+      if (rz < L) slot[rz] = 64 + lane;
+      const int nx = (int)__popcll(__ballot(rx < L));
+      const int nz = L - nx;
+      __syncthreads();
+      if (lane < L) {
+        const int c = slot[lane];
+        my_dn = c < 64 ? sA[c] : sB[c - 64];
+        if (lane == 0) {
+          my_db = dnow;
+          my_pbx = px;
+          my_pby = py;
+          my_ib = idx;
+        } else {
+          const int p = slot[lane - 1];
+          my_db = p < 64 ? sA[p] : sB[p - 64];
+          my_pbx = p < 64 ? sX0[p] : sX1[p - 64];
+          my_pby = p < 64 ? sY0[p] : sY1[p - 64];
+          my_ib = p < 64 ? 0 : 1;
+        }
       }
-      if (k == lane) {
-        my_db = db;
-        my_dn = dnow;
-        my_pbx = pbx;
-        my_pby = pby;
-        my_ib = ib;
+      const int cl = slot[L - 1];
+      dnow = cl < 64 ? sA[cl] : sB[cl - 64];
+      px = cl < 64 ? sX0[cl] : sX1[cl - 64];
+      py = cl < 64 ? sY0[cl] : sY1[cl - 64];
+      idx = cl < 64 ? 0 : 1;
+      dd0 = nx < 64 ? sA[nx] : sA[63] + gd0;
+      i0x = nx < 64 ? sX0[nx] : sX0[63] + g0x;
+      i0y = nx < 64 ? sY0[nx] : sY0[63] + g0y;
+      dd1 = nz < 64 ? sB[nz] : sB[63] + gd1;
+      i1x = nz < 64 ? sX1[nz] : sX1[63] + g1x;
+      i1y = nz < 64 ? sY1[nz] : sY1[63] + g1y;
+      __syncthreads();
+    } else {
+      // plain sequential walk (small distances, axis-aligned lines)
+      for (int k = 0; k < L; k++) {
+        const float db = dnow, pbx = px, pby = py;
+        const int ib = idx;
+        if (dd1 < dd0) {
+          dnow = dd1;
+          idx = 1;
+          dd1 += gd1;
+          px = i1x;
+          py = i1y;
+          i1x += g1x;
+          i1y += g1y;
+        } else {
+          dnow = dd0;
+          idx = 0;
+          px = i0x;
+          py = i0y;
+          dd0 += gd0;
+          i0x += g0x;
+          i0y += g0y;
+        }
+        if (k == lane) {
+          my_db = db;
+          my_dn = dnow;
+          my_pbx = pbx;
+          my_pby = pby;
+          my_ib = ib;
+        }
       }
     }
     z += (float)L * dz;
@@ -260,16 +395,98 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     uint2 ent = make_uint2(0u, 0u);
     if (valid) ent = map[vx + vz * gridx];
     const uint32_t slen = ent.y & 0xffffu;
+    const uint16_t* runs = slab + 2 + ent.x;
+
     // F: projected top of the first drawn element in front of the eye.
-    int F = INT32_MIN;
+    int F = INT32_MIN, kF = -1, ygF = 0;
     uint32_t nel = 0;
+    float texF = 0;
+    int textureF = 0;
+    float ytopF = 0, ybotF = 0;
     {
-      const uint16_t* runs = slab + 2 + ent.x;
       uint32_t w = ent.y >> 16;
       int yg = 0;
+      float tex = 0;
       for (uint32_t k = 0; k < slen; k++) {
         if (k > 0) w = runs[k];
         nel++;
+        int dr = (int)(w >> 10);
+        yg += (int)(w & 1023u) << mip;
+        if (dr == 0) continue;
+        const int texture = (int)tex;
+        tex += (float)dr;
+        dr <<= mip;
+        const float ytop = (float)yg;
+        yg += dr;
+        float zz1 = pz, yy1 = py3;
+        if (mountain + ytop >= 0) {
+          zz1 += czz;
+          yy1 += cyy;
+        }
+        const float pz1 = zz1 + z_add * ytop;
+        if (pz1 <= 0) continue;
+        const float py1 = yy1 + y_add * ytop;
+        F = d_f2i(half + py1 / pz1);
+        kF = (int)k;
+        ygF = yg;
+        texF = tex;
+        textureF = texture;
+        ytopF = ytop;
+        ybotF = (float)yg;
+        break;
+      }
+    }
+
+    // ---- 4. run cache and per-candidate precompute.  The horizon only
+    // rises, so F > ymin now is necessary for acting; the clip-top test is
+    // monotone in ymin only for pz > 0.
+    const bool cand = valid && F > ymin && !(pz > 0.0f && top <= pz * (float)ymin);
+    const uint32_t rc_base = ent.x & ~7u;  // 16-byte aligned start
+    const int rc_delta = (int)(ent.x - rc_base);
+    const uint16_t* cached = rc + lane * kRunCache + rc_delta;
+    const int ccap = cand ? kRunCache - rc_delta : 0;
+    if (cand) {
+      const uint4* src = reinterpret_cast<const uint4*>(slab + rc_base);
+      uint4* dst = reinterpret_cast<uint4*>(rc + lane * kRunCache);
+#pragma unroll
+      for (int i = 0; i < kRunCache / 8; i++) dst[i] = src[i];
+    }
+    // first element's lower end and span parameters (Cuda_Render.h:554-660)
+    int s1F = 0;
+    bool drawF = false;
+    float s1rF = 0, u1zF = 0, u2dzF = 0, onez1F = 0, onedz2F = 0;
+    int F2 = INT32_MIN;
+    if (cand) {
+      float zz2 = pz, yy2 = py3;
+      if (mountain + ybotF < 0) {
+        zz2 += czz;
+        yy2 += cyy;
+      }
+      const float pz2 = zz2 + z_add * ybotF;
+      if (pz2 > 0) {
+        drawF = true;
+        const float py2 = yy2 + y_add * ybotF;
+        s1F = d_f2i(half + py2 / pz2 - 1.0f);
+        const float pz1r = pz + z_add * ytopF;
+        const float py1r = py3 + y_add * ytopF;
+        const float pz2r = pz + z_add * ybotF;
+        const float py2r = py3 + y_add * ybotF;
+        const float s2r = half + py1r / pz1r;
+        s1rF = half + py2r / pz2r;
+        u1zF = texF / pz2r;
+        u2dzF = (float)textureF / pz1r - u1zF;
+        onez1F = 1.0f / pz2r;
+        onedz2F = 1.0f / pz1r - onez1F;
+        u2dzF /= s2r - s1rF;
+        onedz2F /= s2r - s1rF;
+      }
+    }
+    __syncthreads();  // run cache filled
+    if (cand) {  // F2: projected top of the next element the loop tests
+      int yg = ygF;
+      for (uint32_t k = (uint32_t)kF + 1; k < slen; k++) {
+        const int ci = (int)k + 2;
+        const uint32_t w = ci < ccap ? cached[ci] : runs[k];
         int dr = (int)(w >> 10);
         yg += (int)(w & 1023u) << mip;
         if (dr == 0) continue;
@@ -284,12 +501,12 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
         const float pz1 = zz1 + z_add * ytop;
         if (pz1 <= 0) continue;
         const float py1 = yy1 + y_add * ytop;
-        F = d_f2i(half + py1 / pz1);
+        F2 = d_f2i(half + py1 / pz1);
         break;
       }
     }
 
-    // ---- 4. ballot scan over the chunk's steps, in order
+    // ---- 5. ordered replay of the steps that act
     int cur = 0;
     bool done = false;
     for (;;) {
@@ -315,24 +532,77 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
         cnt.rle += r;
       }
       if (!am) break;
-
-      // ---- full element loop of step t (Cuda_Render.h:474-734), uniform
       if (COUNT) {
         cnt.steps++;
         cnt.ptr++;
       }
-      const float tpz = readlane_f(pz, t), tpy = readlane_f(py3, t);
-      const float tczz = readlane_f(czz, t), tcyy = readlane_f(cyy, t);
-      const uint32_t tex_ofs = (uint32_t)readlane_i((int)ent.x, t);
+
+      const uint32_t tofs = (uint32_t)readlane_i((int)ent.x, t);
       const uint32_t tlf = (uint32_t)readlane_i((int)ent.y, t);
       const uint32_t tslen = tlf & 0xffffu;
-      const uint16_t* runs = slab + 2 + tex_ofs;
-      const uint16_t* colours = runs + tslen;
-      uint32_t w = tlf >> 16;
-      int ygen = 0;
-      float tex = 0;
-      for (uint32_t k = 0; k < tslen; k++) {
-        if (k > 0) w = runs[k];
+      const uint16_t* truns = slab + 2 + tofs;
+      const uint16_t* tcol = truns + tslen;
+      const bool tcached = readlane_i((int)cand, t) != 0;
+      const int tdelta = (int)(tofs - (tofs & ~7u));
+      const uint16_t* tcache = rc + t * kRunCache + tdelta;
+      const int tcap = tcached ? kRunCache - tdelta : 0;
+      const float tpz = readlane_f(pz, t), tpy = readlane_f(py3, t);
+      const float tczz = readlane_f(czz, t), tcyy = readlane_f(cyy, t);
+
+      // element loop state after the first drawn element in front
+      uint32_t k0;
+      int ygen;
+      float tex;
+      const bool fast = !COUNT && tcached;
+      if (fast) {
+        // ---- first drawn element: precomputed (Cuda_Render.h:540-733)
+        const int s2 = readlane_i(F, t);  // > ymin by the ballot
+        const bool tdraw = readlane_i((int)drawF, t) != 0;
+        int s1 = readlane_i(s1F, t);
+        if (tdraw && s1 < ymax) {
+          int e2 = s2;
+          if (e2 >= ymax) {
+            e2 = ymax;
+            ymax = s1;
+          }
+          if (s1 <= ymin) {
+            s1 = ymin;
+            ymin = e2;
+            if (ymax > ymin) ymin = first_clear(bits, ymin, ymax, lane);
+          }
+          const int y0 = first_clear(bits, s1, e2, lane);
+          if (y0 < e2) {
+            const float s1r = readlane_f(s1rF, t);
+            const float u1z = readlane_f(u1zF, t), u2dz = readlane_f(u2dzF, t);
+            const float onez1 = readlane_f(onez1F, t), onedz2 = readlane_f(onedz2F, t);
+            const float mult = (float)(y0 + 1) - s1r;
+            const float uzb = u1z + u2dz * mult;
+            const float ozb = onez1 + onedz2 * mult;
+            const float thi = readlane_f(texF, t) - 1.0f;
+            const float tlo = (float)readlane_i(textureF, t);
+            draw_span(y0, e2, uzb, ozb, u2dz, onedz2, tlo, thi, tcache, tcap, tcol,
+                      2 + (int)tslen, row, bits, lane);
+          }
+        }
+        // the next element tested breaks unless it reaches above ymin
+        if (readlane_i(F2, t) <= ymin) {
+          cur = t + 1;
+          continue;
+        }
+        k0 = (uint32_t)readlane_i(kF, t) + 1;
+        ygen = readlane_i(ygF, t);
+        tex = readlane_f(texF, t);
+      } else {
+        k0 = 0;
+        ygen = 0;
+        tex = 0;
+      }
+
+      // ---- remaining elements (Cuda_Render.h:511-734), uniform
+      for (uint32_t k = k0; k < tslen; k++) {
+        uint32_t w;
+        if (k == 0) w = tlf >> 16;
+        else w = ((int)k + 2 < tcap) ? tcache[k + 2] : truns[k];
         if (COUNT) {
           cnt.elems++;
           if (k > 0) cnt.rle++;
@@ -390,35 +660,12 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
         u2dz /= s2r - s1r;
         onedz2 /= s2r - s1r;
         const float mult = (float)(y0 + 1) - s1r;
-        float uzb = u1z + u2dz * mult;
-        float ozb = onez1 + onedz2 * mult;
+        const float uzb = u1z + u2dz * mult;
+        const float ozb = onez1 + onedz2 * mult;
         if (COUNT) cnt.spans++;
-        const float tlo = (float)texture, thi = tex - 1.0f;
-        // ---- 5. span, 64 pixels per pass
-        for (int base = y0; base < s2; base += 64) {
-          float uz = uzb, oz = ozb;
-          // pixel base+lane has had `lane` additions; lanes past the span
-          // end stop early (their values are not used)
-          const int adds = min(lane, s2 - 1 - base);
-          for (int i = 0; i < adds; i++) {
-            uz += u2dz;
-            oz += onedz2;
-          }
-          const int y = base + lane;
-          const bool draw = y < s2 && !lds_bit(bits, y);
-          if (draw) {
-            const float fu = fminf(fmaxf(uz / oz, tlo), thi);
-            const uint32_t u = (uint32_t)d_f2i(fu);
-            const uint32_t depth = (uint32_t)d_f2i(1.0f / oz) & 0xfffeu;
-            row[y] = (uint32_t)colours[u] + (depth << 16);
-          }
-          if (COUNT) cnt.col += (uint64_t)__popcll(__ballot(draw));
-          __syncthreads();
-          if (draw) atomicOr(&bits[y >> 5], 1u << (y & 31));
-          __syncthreads();
-          uzb = readlane_f(uz, 63) + u2dz;
-          ozb = readlane_f(oz, 63) + onedz2;
-        }
+        const int d = draw_span(y0, s2, uzb, ozb, u2dz, onedz2, (float)texture, tex - 1.0f, tcache,
+                                tcap, tcol, 2 + (int)tslen, row, bits, lane);
+        if (COUNT) cnt.col += (uint64_t)d;
       }
       cur = t + 1;
     }

@@ -1,0 +1,61 @@
+// Checks ffadd() (the product's closed form of repeated fp32 addition)
+// against k plain additions.  Built and run by tests/test_ffadd.py.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+
+#include "../../rle-based-voxel-raycasting_amd/csrc/ffadd.h"
+
+static float seq(float s, float g, int k) {
+  volatile float a = s;
+  for (int i = 0; i < k; i++) a = a + g;
+  return a;
+}
+
+static bool same(float x, float y) { return std::memcmp(&x, &y, 4) == 0; }
+
+int main(int argc, char** argv) {
+  long n = argc > 1 ? atol(argv[1]) : 2000000;
+  std::mt19937_64 rng(12345);
+  std::uniform_real_distribution<double> U(0.0, 1.0);
+  long bad = 0, ties = 0;
+  for (long t = 0; t < n; t++) {
+    const int kind = (int)(t % 5);
+    float s, g;
+    if (kind == 0) {  // DDA-like: distances and gradients
+      s = (float)(U(rng) * std::pow(2.0, U(rng) * 20.0 - 3.0));
+      g = (float)(std::pow(2.0, U(rng) * 8.0 - 2.0));
+    } else if (kind == 1) {  // exact half-ulp remainders (ties)
+      const int e = (int)(U(rng) * 16);
+      s = (float)std::ldexp(1.0 + U(rng), e);
+      const double u = std::ldexp(1.0, e - 23);
+      const long odd = 2 * (long)(U(rng) * 50) + 1;
+      g = (float)(u * 0.5 * (double)odd);
+      ties++;
+    } else if (kind == 2) {  // integer steps with fractional start (positions)
+      s = (float)(U(rng)) + (float)(int)(U(rng) * 5000.0);
+      g = (float)std::ldexp(1.0, (int)(U(rng) * 7));
+    } else if (kind == 3) {  // tiny increments (absorption)
+      s = (float)(U(rng) * 1e6);
+      g = (float)(U(rng) * 1e-3);
+    } else {  // zero start
+      s = 0.0f;
+      g = (float)(U(rng) * 3.0 + 1e-3);
+    }
+    if (U(rng) < 0.5) {
+      s = -s;
+      g = -g;
+    }
+    const int k = (int)(U(rng) * 300);
+    const float want = seq(s, g, k);
+    const float got = rlevox::ffadd(s, g, k);
+    if (!same(want, got)) {
+      if (bad < 10)
+        std::printf("MISMATCH s=%a g=%a k=%d want=%a got=%a\n", s, g, k, want, got);
+      bad++;
+    }
+  }
+  std::printf("checked %ld (ties %ld) mismatches %ld\n", n, ties, bad);
+  return bad ? 1 : 0;
+}
