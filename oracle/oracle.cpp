@@ -787,6 +787,9 @@ static inline void compose_pixel(const rlevox_raymap& rm, const uint32_t* raybuf
     // seg_up_x / seg_dn_x (colorize_buddha_soft.frag:68-77)
     const bool first = upper ? !rxg : rxg;
     tx = first ? ob : 1.0f - ob;
+#if ORC_PLANT == 4
+    tx = 1.0f - tx;
+#endif
   } else {
     const float t = left ? 1.0f : 0.0f;
     float ang3 = scy1 * std::fabs(1.0f - t - vx) / scx1;
@@ -875,7 +878,11 @@ void orc_gapfill(int W, int H, const uint16_t* cin, const int32_t* dep, uint16_t
         for (int b = 0; b < 5; b++) {
           const int oy = floor_div((2 * b - 5) * r_q8 + 640, 1280);
           int xx = std::min(std::max(x + ox, 0), W - 1);
+#if ORC_PLANT == 5
+          int yy = std::min(std::max(row + oy, 0), H - 1);
+#else
           int yy = std::min(std::max(row - oy, 0), H - 1);
+#endif
           const size_t t = (size_t)yy * W + xx;
           const int d = dep[t];
           if (d < RLEVOX_SKY_DEPTH && (int64_t)10 * dmin >= (int64_t)7 * d) {

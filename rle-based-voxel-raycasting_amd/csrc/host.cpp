@@ -77,9 +77,9 @@ static rlevox_status build_ptrmap(const uint16_t* slabs, uint64_t nslabs, int sx
 }
 
 // Checks everything the raycast kernel relies on to stay inside its arrays:
-// power-of-two grids halving per mip, every column inside the slab array,
-// the cached count/first element equal to the slab's, and as many colours as
-// drawn voxels.
+// power-of-two grids halving per mip (heights halve too), every column inside
+// the slab array, the cached count/first element equal to the slab's, as many
+// colours as drawn voxels and no runs past the column height.
 static rlevox_status validate(const rlevox_mip* mips, int nmips) {
   if (!mips || nmips < 1 || nmips > RLEVOX_MAX_MIPS)
     return fail(RLEVOX_ERR_INVALID_ARG, "scene: need 1..%d mips, got %d", RLEVOX_MAX_MIPS, nmips);
@@ -92,7 +92,9 @@ static rlevox_status validate(const rlevox_mip* mips, int nmips) {
     if (mp.sx != (sx0 >> m) || mp.sz != (sz0 >> m) || mp.sx < 1 || mp.sz < 1)
       return fail(RLEVOX_ERR_INVALID_SCENE, "scene: mip %d grid %dx%d, expected %dx%d", m, mp.sx,
                   mp.sz, sx0 >> m, sz0 >> m);
-    if (mp.sy < 1) return fail(RLEVOX_ERR_INVALID_SCENE, "scene: mip %d height %d", m, mp.sy);
+    if (mp.sy < 1 || (m > 0 && mp.sy != std::max(1, mips[0].sy >> m)))
+      return fail(RLEVOX_ERR_INVALID_SCENE, "scene: mip %d height %d, expected %d", m, mp.sy,
+                  std::max(1, mips[0].sy >> m));
     if (!mp.slabs || !mp.ptrmap)
       return fail(RLEVOX_ERR_INVALID_ARG, "scene: mip %d has a null array", m);
     const int64_t ncols = (int64_t)mp.sx * mp.sz;
@@ -112,12 +114,14 @@ static rlevox_status validate(const rlevox_mip* mips, int nmips) {
           else if (ofs + 2 + count + ncol > mp.nslabs) reason = 3;
           else if (count > 0 && (lf >> 16) != mp.slabs[ofs + 2]) reason = 4;
           else {
-            uint64_t drawn = 0;
+            uint64_t drawn = 0, height = 0;
             for (uint32_t e = 0; e < count; e++) {
               const uint16_t w = e == 0 ? (uint16_t)(lf >> 16) : mp.slabs[ofs + 2 + e];
               drawn += w >> 10;
+              height += (w >> 10) + (w & 1023u);
             }
             if (drawn != ncol) reason = 5;
+            else if (height > (uint64_t)mp.sy) reason = 6;
           }
         }
         if (reason) {
@@ -131,7 +135,8 @@ static rlevox_status validate(const rlevox_mip* mips, int nmips) {
       static const char* msg[] = {"", "header outside the slab array", "count differs from the pointer map",
                                   "runs or colours outside the slab array",
                                   "first element differs from the pointer map",
-                                  "colour count differs from the drawn voxels"};
+                                  "colour count differs from the drawn voxels",
+                                  "runs extend past the column height"};
       return fail(RLEVOX_ERR_INVALID_SCENE, "scene: mip %d column %lld: %s", m,
                   (long long)bad.load(), msg[why.load()]);
     }

@@ -201,17 +201,40 @@ def raymap(pose, W, H, mip_distance=0, far_distance=0):
     return rm
 
 
-class HostScene:
-    """A scene in host memory; `mips` are numpy views of the native arrays."""
+class _HostHandle:
+    """Owns a native rlevox_hscene; freed when the last reference goes."""
 
     def __init__(self, handle):
         self.handle = C.c_void_p(handle)
+
+    def __del__(self):
+        if self.handle:
+            lib.rlevox_hscene_free(self.handle)
+            self.handle = None
+
+
+def _owned_array(owner, addr, ctype, dtype, n):
+    """A numpy view of native memory that keeps `owner` alive."""
+    if n == 0:
+        return np.zeros(0, dtype)
+    buf = (ctype * n).from_address(addr)
+    buf._owner = owner
+    return np.frombuffer(buf, dtype)
+
+
+class HostScene:
+    """A scene in host memory; `mips` are numpy views of the native arrays
+    (each view keeps the native scene alive)."""
+
+    def __init__(self, handle):
+        self._native = _HostHandle(handle)
+        self.handle = self._native.handle
         self.mips = []
         for m in range(lib.rlevox_hscene_nmips(self.handle)):
             mp = Mip()
             check(lib.rlevox_hscene_mip(self.handle, m, C.byref(mp)))
-            ptr = np.ctypeslib.as_array(C.cast(mp.ptrmap, P(C.c_uint32)), shape=(2 * mp.sx * mp.sz,))
-            slabs = np.ctypeslib.as_array(C.cast(mp.slabs, P(C.c_uint16)), shape=(mp.nslabs,))
+            ptr = _owned_array(self._native, mp.ptrmap, C.c_uint32, np.uint32, 2 * mp.sx * mp.sz)
+            slabs = _owned_array(self._native, mp.slabs, C.c_uint16, np.uint16, mp.nslabs)
             self.mips.append({"sx": mp.sx, "sy": mp.sy, "sz": mp.sz, "ptrmap": ptr, "slabs": slabs})
 
     def mip_array(self):
@@ -222,16 +245,10 @@ class HostScene:
         return sum(m["ptrmap"].nbytes + m["slabs"].nbytes for m in self.mips)
 
     def close(self):
-        if self.handle:
-            self.mips = []
-            lib.rlevox_hscene_free(self.handle)
-            self.handle = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        """Drops this object's references; the memory goes with the last view."""
+        self.mips = []
+        self._native = None
+        self.handle = None
 
 
 def mip_array(mips):

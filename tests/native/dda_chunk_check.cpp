@@ -1,7 +1,7 @@
 // Emulates the wavefront DDA chunk (64 lanes, each taking one step via
 // ffadd(), ordered by merge ranks of the two crossing sequences) and checks
 // it against the sequential DDA of Cuda_Render.h:376-414, including the
-// gradient doubling at mip switches.  Built and run by tests/test_ffadd.py.
+// gradient doubling at mip switches.  Built and run by tests/test_native_checks.py.
 #include <cmath>
 #include <cstdio>
 #include <cstring>

@@ -1,5 +1,5 @@
 // Checks ffadd() (the product's closed form of repeated fp32 addition)
-// against k plain additions.  Built and run by tests/test_ffadd.py.
+// against k plain additions.  Built and run by tests/test_native_checks.py.
 #include <cmath>
 #include <cstdio>
 #include <cstring>

@@ -159,3 +159,24 @@ def test_work_counters_match_oracle(ctx, oracle, c1_scene, c1_dev):
         ctx.set_tuning()
         d_rb.free()
         d_clip.free()
+
+
+def test_golden_digests(ctx, c1_scene, tiny_scene, c1_dev, tiny_dev):
+    """Device renders reproduce the committed golden digests (independent of
+    the oracle build on this machine)."""
+    import json
+    import os
+    import sys
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_golden as G
+    golden = json.load(open(os.path.join(here, "golden", "golden.json")))
+    dev = {"c1": c1_dev, "tiny": tiny_dev}
+    for case in golden["renders"]:
+        pose = rlevox.Pose.make(case["pose"][:3], case["pose"][3:])
+        rm = rlevox.raymap(pose, case["W"], case["H"], case["mip_distance"], case["far_distance"])
+        got = gpu_stages(ctx, dev[case["scene"]], rm)
+        n = max(rm.map_line_count, 1)
+        assert G.raybuf_digest(got["raybuf"][:n], got["clip"][:n]) == case["raybuf"], case["name"]
+        assert G.digest(got["compose"], got["depth"]) == case["compose"], case["name"]
+        assert G.digest(got["colour"]) == case["colour"], case["name"]
