@@ -58,6 +58,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=1500)
+    ap.add_argument("--serve-rates", type=int, nargs="*", default=[500, 1000, 2000])
+    ap.add_argument("--serve-batch", type=int, default=48)
+    ap.add_argument("--serve-streams", type=int, default=3)
+    ap.add_argument("--serve-budget-gb", type=int, default=48)
     ap.add_argument("--seed", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -153,26 +159,38 @@ def main():
     }
 
     if not args.no_batch:
+        for b in (d_rb, d_clip, d_c0, d_col, d_dep):
+            b.free()
+        d_rb = d_clip = d_c0 = d_col = d_dep = None
+        scenes_h = [hs] + [workloads.gen("c2", seed=s) for s in (2, 3, 4)]
+        log("generated c2 scenes 2-4")
+        scenes = [scene] + [ctx.upload(h.mips) for h in scenes_h[1:]]
         try:
-            result["batched"] = bench_batched(ctx, rlevox, workloads, args)
+            result["batched"] = bench_batched(ctx, rlevox, workloads, scenes_h, scenes)
         except Exception as e:  # reported, not fatal: the plain line stands
             log("batched leg failed: %s" % e)
             result["batched"] = {"error": str(e)}
+        if not args.no_serve:
+            try:
+                result["served"] = bench_served(ctx, rlevox, workloads, scenes_h, scenes, args)
+            except Exception as e:
+                log("served leg failed: %s" % e)
+                result["served"] = {"error": str(e)}
+        for sc in scenes[1:]:
+            sc.free()
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(hs, poses, W, H, args.cpu_seconds)
 
     for b in (d_rb, d_clip, d_c0, d_col, d_dep):
-        b.free()
+        if b is not None:
+            b.free()
     scene.free()
     ctx.close()
     print(json.dumps(result), flush=True)
 
 
-def bench_batched(ctx, rlevox, workloads, args):
-    scenes_h = [workloads.gen("c2", seed=s) for s in (1, 2, 3, 4)]
-    log("generated 4 c2 scenes")
-    scenes = [ctx.upload(h.mips) for h in scenes_h]
+def bench_batched(ctx, rlevox, workloads, scenes_h, scenes):
     views_spec = workloads.c4_views([h.mips[0] for h in scenes_h], n=64, seed=4)
     bufs = []
     views = []
@@ -215,7 +233,114 @@ def bench_batched(ctx, rlevox, workloads, args):
     log("batched: %s" % out)
     for b in bufs:
         b.free()
-    for s in scenes:
+    return out
+
+
+C5_SIZES = [((640, 480), 0.5), ((1920, 1080), 0.35), ((3840, 2160), 0.15)]
+
+
+def bench_served(ctx, rlevox, workloads, scenes_h, scenes, args):
+    """Config 5: 12 resident scenes, Poisson arrivals, mixed resolutions.
+
+    The 12 resident scenes are the 4 generated c2 scenes uploaded 3 times
+    each (distinct device copies).  Requests draw a scene, a seeded pose and a
+    resolution (640x480 / 1920x1080 / 3840x2160 at 0.5 / 0.35 / 0.15) and are
+    rendered into device framebuffers.  Three runs over the same requests:
+    plain (one synchronous rlevox_render per request, back to back), served
+    open-loop at the given arrival rate (latency from submit to framebuffer
+    ready), and served saturated (all submitted at once: peak throughput).
+    """
+    import threading
+    resident = list(scenes) + [ctx.upload(h.mips) for h in scenes_h for _ in range(2)]
+    rng = np.random.default_rng(55)
+    n = args.serve_requests
+    reqs = []
+    for i in range(n):
+        u = rng.random()
+        (W, H) = C5_SIZES[0][0] if u < 0.5 else (C5_SIZES[1][0] if u < 0.85 else C5_SIZES[2][0])
+        si = int(rng.integers(len(resident)))
+        pose = workloads.random_views(scenes_h[si % len(scenes_h)].mips[0], 1,
+                                      int(rng.integers(1 << 30)))[0]
+        reqs.append((si, pose, W, H))
+    offs, total = [], 0
+    for _, _, W, H in reqs:
+        offs.append(total)
+        total += (W * H * 6 + 255) // 256 * 256
+    arena = ctx.alloc(total)
+    base = arena.ptr.value
+    dcol = [base + o for o in offs]
+    ddep = [base + o + (W * H * 2 + 255) // 256 * 256 for o, (_, _, W, H) in zip(offs, reqs)]
+    rays = sum(W * H for _, _, W, H in reqs)
+    log("served: %d requests, %.1f Mrays, %d resident scenes (%.1f GB)" % (
+        n, rays / 1e6, len(resident), sum(s.device_bytes for s in resident) / 1e9))
+
+    # plain per-request path
+    stream = ctx.stream()
+    for k in range(min(20, n)):
+        si, p, W, H = reqs[k]
+        ctx.render(resident[si], p, W, H, _Buf(dcol[k]), _Buf(ddep[k]), stream=stream)
+    t0 = time.perf_counter()
+    lat_plain = []
+    for k, (si, p, W, H) in enumerate(reqs):
+        t1 = time.perf_counter()
+        ctx.render(resident[si], p, W, H, _Buf(dcol[k]), _Buf(ddep[k]), stream=stream)
+        lat_plain.append((time.perf_counter() - t1) * 1e3)
+    plain_s = time.perf_counter() - t0
+    out = {"requests": n, "resident_scenes": len(resident), "rays": rays,
+           "mix": "640x480:0.5 1920x1080:0.35 3840x2160:0.15",
+           "plain_mrays_s": round(rays / plain_s / 1e6, 2)}
+
+    def run(rate):
+        srv = rlevox.Server(ctx, memory_budget=args.serve_budget_gb << 30,
+                            max_batch_views=args.serve_batch, max_queue=1 << 20,
+                            streams=args.serve_streams)
+        tickets = [None] * n
+        t_start = time.perf_counter()
+        if rate is None:
+            for k, (si, p, W, H) in enumerate(reqs):
+                tickets[k] = srv.submit(resident[si], p, W, H, dcol[k], ddep[k])
+        else:
+            arrivals = np.cumsum(np.random.default_rng(77).exponential(1.0 / rate, n))
+
+            def submitter():
+                for k, (si, p, W, H) in enumerate(reqs):
+                    d = arrivals[k] - (time.perf_counter() - t_start)
+                    if d > 0:
+                        time.sleep(d)
+                    tickets[k] = srv.submit(resident[si], p, W, H, dcol[k], ddep[k])
+            th = threading.Thread(target=submitter)
+            th.start()
+        lat = []
+        for k in range(n):
+            while tickets[k] is None:
+                time.sleep(0.0002)
+            lat.append(srv.wait(tickets[k])[2])
+        wall = time.perf_counter() - t_start
+        if rate is not None:
+            th.join()
+        st = srv.stats()
+        srv.close()
+        lat = np.array(lat)
+        r1080 = np.array([W == 1920 for _, _, W, _ in reqs])
+        return {"mrays_s": round(rays / wall / 1e6, 2), "wall_s": round(wall, 3),
+                "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                "p99_1080p_ms": round(float(np.percentile(lat[r1080], 99)), 3),
+                "batches": int(st["batches"]), "busy_s": round(st["busy_seconds"], 3)}
+
+    run(None)  # warm-up: lane graphs, scratch growth
+    sat = run(None)
+    out["saturated"] = sat
+    out["speedup_vs_plain"] = round(sat["mrays_s"] / out["plain_mrays_s"], 3)
+    out["open_loop"] = {}
+    for rate in args.serve_rates:
+        r = run(float(rate))
+        out["open_loop"][str(rate)] = r
+        log("served at %s req/s: %s" % (rate, r))
+    out["plain_latency_p50_ms"] = round(float(np.percentile(lat_plain, 50)), 3)
+    log("served: %s" % {k: v for k, v in out.items() if k != "open_loop"})
+    arena.free()
+    for s in resident[len(scenes):]:
         s.free()
     return out
 

@@ -286,15 +286,23 @@ rlevox_status rlevox_autotune(rlevox_ctx* ctx, const rlevox_view* views, int32_t
 
 rlevox_status rlevox_server_create(rlevox_ctx* ctx, const rlevox_server_config* cfg,
                                    rlevox_server** out);
-/* Queues one request; *ticket identifies it.  QUEUE_FULL when max_queue
- * requests are pending, OVER_BUDGET when the request alone exceeds the
- * memory budget. */
+/* Queues one request; *ticket identifies it.  With d_colour and d_depth
+ * (W*H device arrays, both or neither) the framebuffer is rendered straight
+ * into them and stays on the device; without them the server renders into
+ * its own arena and copies the result to host memory for rlevox_server_wait.
+ * QUEUE_FULL when max_queue requests are pending, OVER_BUDGET when the
+ * request alone exceeds the memory budget (ray buffer, line clip ranges,
+ * gap-fill temporary and, for host delivery, the framebuffer). */
 rlevox_status rlevox_server_submit(rlevox_server* srv, const rlevox_scene* scene,
                                    const rlevox_pose* pose, int32_t W, int32_t H,
-                                   uint64_t* ticket);
-/* Waits up to timeout_ms (-1 = forever) for the request; copies its
- * framebuffer to the host arrays (may be NULL) and its latency (arrival to
- * framebuffer ready) to *latency_ms.  NOT_READY on time-out. */
+                                   uint16_t* d_colour, int32_t* d_depth, uint64_t* ticket);
+/* Waits up to timeout_ms (-1 = forever) for the request and consumes its
+ * ticket; copies a host-delivered framebuffer to the host arrays (may be
+ * NULL; ignored for device delivery) and the latency (submit to framebuffer
+ * ready: on the device for device delivery, in host memory otherwise) to
+ * *latency_ms.  NOT_READY on time-out (the ticket stays valid), CANCELLED
+ * for a cancelled request.  No call on a server may be in progress when it
+ * is destroyed. */
 rlevox_status rlevox_server_wait(rlevox_server* srv, uint64_t ticket, int32_t timeout_ms,
                                  uint16_t* h_colour, int32_t* h_depth, double* latency_ms);
 rlevox_status rlevox_server_cancel(rlevox_server* srv, uint64_t ticket);

@@ -18,8 +18,8 @@ OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librlevox.so")
 ARCH = os.environ.get("RLEVOX_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "api.hip", "batch.hip"]
-HOST_SOURCES = ["host.cpp", "raymap.cpp", "scene.cpp", "pending.cpp"]
+HIP_SOURCES = ["kernels.hip", "api.hip", "batch.hip", "server.hip", "tune.hip"]
+HOST_SOURCES = ["host.cpp", "raymap.cpp", "scene.cpp"]
 
 FP_FLAGS = ["-ffp-contract=off", "-fno-fast-math"]
 HIPCC_FLAGS = [

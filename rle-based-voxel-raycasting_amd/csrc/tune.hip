@@ -1,0 +1,95 @@
+// Autotuner: times candidate kernel configurations on sample views and keeps
+// the fastest.  The reference fixes its launch shape at compile time
+// (Cuda_Main.cu:163-178: 64-thread blocks, one thread per ray line); here
+// the shape is a runtime choice per context, and every candidate produces
+// bit-identical framebuffers, so the choice only affects speed.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "api_internal.h"
+
+using namespace rlevox;
+
+namespace {
+
+// Candidates: one wavefront per line (lanes take consecutive DDA steps), and
+// one lane per line in blocks of 16..256 lanes.
+const rlevox_tuning kCandidates[] = {
+    {1, 1, 0, 1},  {64, 1, 0, 1}, {64, 2, 0, 1}, {64, 4, 0, 1},
+    {32, 1, 0, 1}, {32, 2, 0, 1}, {16, 1, 0, 1}, {16, 4, 0, 1},
+};
+
+rlevox_status time_views(rlevox_ctx* ctx, const rlevox_view* views, int n,
+                         const rlevox_render_params* params, hipStream_t s, hipEvent_t a,
+                         hipEvent_t b, int reps, double* ms) {
+  hipError_t e = hipEventRecord(a, s);
+  if (e != hipSuccess) return hip_fail(e, "autotune event");
+  for (int r = 0; r < reps; r++)
+    for (int i = 0; i < n; i++) {
+      const rlevox_view& v = views[i];
+      rlevox_status st = rlevox_render(ctx, v.scene, &v.pose, v.W, v.H, params, v.d_colour,
+                                       v.d_depth, s);
+      if (st != RLEVOX_OK) return st;
+    }
+  e = hipEventRecord(b, s);
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  float t = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, a, b);
+  if (e != hipSuccess) return hip_fail(e, "autotune timing");
+  *ms = (double)t / reps;
+  return RLEVOX_OK;
+}
+
+}  // namespace
+
+extern "C" rlevox_status rlevox_autotune(rlevox_ctx* ctx, const rlevox_view* views, int32_t nviews,
+                                         const rlevox_render_params* params, rlevox_tuning* best,
+                                         double* best_ms) {
+  clear_error();
+  rlevox_status st = ctx_bind(ctx);
+  if (st != RLEVOX_OK) return st;
+  if (!views || nviews < 1 || !best)
+    return fail(RLEVOX_ERR_INVALID_ARG, "autotune: need views and an output");
+  for (int i = 0; i < nviews; i++)
+    if (!views[i].scene || !views[i].d_colour || !views[i].d_depth)
+      return fail(RLEVOX_ERR_INVALID_ARG, "autotune: view %d has a null pointer", i);
+  rlevox_tuning saved;
+  rlevox_get_tuning(ctx, &saved);
+  hipStream_t s = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  if (e != hipSuccess) {
+    st = hip_fail(e, "autotune setup");
+  } else {
+    double best_t = 0;
+    int best_i = -1;
+    for (int c = 0; c < (int)(sizeof(kCandidates) / sizeof(kCandidates[0])); c++) {
+      rlevox_tuning t = kCandidates[c];
+      t.order = saved.order;
+      if ((st = rlevox_set_tuning(ctx, &t)) != RLEVOX_OK) break;
+      double ms = 0;
+      // one untimed pass (code-object load, scratch growth), then the timed ones
+      if ((st = time_views(ctx, views, nviews, params, s, a, b, 1, &ms)) != RLEVOX_OK) break;
+      if ((st = time_views(ctx, views, nviews, params, s, a, b, 2, &ms)) != RLEVOX_OK) break;
+      if (best_i < 0 || ms < best_t) {
+        best_t = ms;
+        best_i = c;
+      }
+    }
+    if (st == RLEVOX_OK) {
+      *best = kCandidates[best_i];
+      best->order = saved.order;
+      if (best_ms) *best_ms = best_t;
+      st = rlevox_set_tuning(ctx, best);
+    } else {
+      rlevox_set_tuning(ctx, &saved);
+    }
+  }
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  if (s) hipStreamDestroy(s);
+  return st;
+}

@@ -153,7 +153,8 @@ _sigs = {
     "rlevox_get_tuning": (C.c_int, [_vp, P(Tuning)]),
     "rlevox_autotune": (C.c_int, [_vp, P(View), C.c_int32, P(RenderParams), P(Tuning), P(C.c_double)]),
     "rlevox_server_create": (C.c_int, [_vp, P(ServerConfig), P(_vp)]),
-    "rlevox_server_submit": (C.c_int, [_vp, _vp, P(Pose), C.c_int32, C.c_int32, P(C.c_uint64)]),
+    "rlevox_server_submit": (C.c_int, [_vp, _vp, P(Pose), C.c_int32, C.c_int32, _vp, _vp,
+                                       P(C.c_uint64)]),
     "rlevox_server_wait": (C.c_int, [_vp, C.c_uint64, C.c_int32, _vp, _vp, P(C.c_double)]),
     "rlevox_server_cancel": (C.c_int, [_vp, C.c_uint64]),
     "rlevox_server_stats_get": (C.c_int, [_vp, P(ServerStats)]),
@@ -445,6 +446,64 @@ class Context:
         check(lib.rlevox_autotune(self.handle, arr, len(views), C.byref(prm), C.byref(best),
                                   C.byref(ms)))
         return best, ms.value
+
+
+class Server:
+    """Continuous-batching render loop (rlevox_server_*).  submit() returns a
+    ticket; wait() returns (colour, depth, latency_ms) as numpy arrays."""
+
+    def __init__(self, ctx, memory_budget=8 << 30, max_batch_views=64, max_queue=4096, streams=2,
+                 params=None):
+        cfg = ServerConfig(memory_budget, max_batch_views, max_queue, streams, 0,
+                           params or RenderParams(0, 0, 0, 0))
+        h = C.c_void_p()
+        check(lib.rlevox_server_create(ctx.handle, C.byref(cfg), C.byref(h)))
+        self.handle = h
+        self.ctx = ctx
+        self.sizes = {}
+
+    def submit(self, scene, pose, W, H, d_col=None, d_dep=None):
+        """d_col/d_dep: DeviceBuffers (or raw device addresses) for device
+        delivery; omitted, the framebuffer comes back to the host."""
+        t = C.c_uint64()
+        addr = lambda b: None if b is None else (b if isinstance(b, int) else b.ptr.value)
+        check(lib.rlevox_server_submit(self.handle, scene.handle, C.byref(pose), W, H,
+                                       addr(d_col), addr(d_dep), C.byref(t)))
+        self.sizes[t.value] = (W, H) if d_col is None else (0, 0)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1, fetch=True):
+        W, H = self.sizes.get(ticket, (0, 0))
+        if W == 0:
+            fetch = False  # device delivery, or unknown here (the library reports it)
+        col = np.empty((H, W), np.uint16) if fetch else None
+        dep = np.empty((H, W), np.int32) if fetch else None
+        lat = C.c_double()
+        st = lib.rlevox_server_wait(self.handle, ticket, timeout_ms, _ptr(col), _ptr(dep),
+                                    C.byref(lat))
+        if st != 7:  # NOT_READY keeps the ticket
+            self.sizes.pop(ticket, None)
+        check(st)
+        return col, dep, lat.value
+
+    def cancel(self, ticket):
+        check(lib.rlevox_server_cancel(self.handle, ticket))
+
+    def stats(self):
+        s = ServerStats()
+        check(lib.rlevox_server_stats_get(self.handle, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.handle:
+            lib.rlevox_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def make_view(scene, pose, W, H, d_col, d_dep):
