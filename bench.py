@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=3)
     ap.add_argument("--serve-budget-gb", type=int, default=48)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run a plain copy kernel over MIB MiB (for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -81,6 +83,17 @@ def main():
     poses = workloads.flythrough(hs.mips[0], frames=100, seed=7)
 
     ctx = rlevox.Context(0)
+    if args.calibrate_copy:
+        nb = args.calibrate_copy << 20
+        src, dst = ctx.alloc(nb), ctx.alloc(nb)
+        src.fill_u32(0x01020304)
+        for _ in range(2):
+            dst.copy_from(src)
+        ctx.sync()
+        log("copy calibration: 2 x k_copy_u128 over %d MiB (read %d B + write %d B each)"
+            % (args.calibrate_copy, nb, nb))
+        src.free()
+        dst.free()
     scene = ctx.upload(hs.mips)
     log("uploaded %.2f GB to HBM" % (scene.device_bytes / 1e9))
     stream = ctx.stream()

@@ -314,6 +314,10 @@ void rlevox_server_destroy(rlevox_server* srv);
 rlevox_status rlevox_dev_alloc(rlevox_ctx* ctx, uint64_t bytes, void** out);
 rlevox_status rlevox_dev_free(rlevox_ctx* ctx, void* p);
 rlevox_status rlevox_dev_fill_u32(rlevox_ctx* ctx, void* p, uint32_t value, uint64_t count, void* stream);
+/* Device-to-device copy by a plain streaming kernel (16-byte aligned
+ * pointers, size a multiple of 16): exact read+write traffic, used to
+ * calibrate hardware byte counters.  No counterpart in the reference. */
+rlevox_status rlevox_dev_copy(rlevox_ctx* ctx, void* dst, const void* src, uint64_t bytes, void* stream);
 rlevox_status rlevox_memcpy_htod(rlevox_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 rlevox_status rlevox_memcpy_dtoh(rlevox_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 rlevox_status rlevox_stream_create(rlevox_ctx* ctx, void** out);

@@ -303,6 +303,18 @@ rlevox_status rlevox_dev_fill_u32(rlevox_ctx* ctx, void* p, uint32_t value, uint
   return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "fill");
 }
 
+rlevox_status rlevox_dev_copy(rlevox_ctx* ctx, void* dst, const void* src, uint64_t bytes,
+                              void* stream) {
+  clear_error();
+  rlevox_status st = ctx_bind(ctx);
+  if (st != RLEVOX_OK) return st;
+  if ((bytes & 15) || (((uintptr_t)dst | (uintptr_t)src) & 15))
+    return fail(RLEVOX_ERR_INVALID_ARG, "dev_copy: pointers and size must be multiples of 16");
+  if (bytes && (!dst || !src)) return fail(RLEVOX_ERR_INVALID_ARG, "dev_copy: null pointer");
+  hipError_t e = launch_copy_u128(dst, src, bytes / 16, (hipStream_t)stream);
+  return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "copy");
+}
+
 rlevox_status rlevox_memcpy_htod(rlevox_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
   clear_error();
   rlevox_status st = ctx_bind(ctx);

@@ -85,6 +85,7 @@ struct BatchSlot {
 };
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t count, hipStream_t s);
+hipError_t launch_copy_u128(void* dst, const void* src, uint64_t n16, hipStream_t s);
 
 }  // namespace rlevox
 

@@ -113,6 +113,24 @@ __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* __restrict__ p, uint
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
+// Copies n 16-byte words: a plain streaming kernel of known traffic, used to
+// calibrate hardware byte counters against an exact read+write volume.
+__global__ __launch_bounds__(256) void k_copy_u128(uint4* __restrict__ dst,
+                                                  const uint4* __restrict__ src, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy_u128(void* dst, const void* src, uint64_t n16, hipStream_t s) {
+  if (n16 == 0) return hipSuccess;
+  uint64_t blocks = (n16 + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_copy_u128, dim3((unsigned)blocks), dim3(256), 0, s, (uint4*)dst,
+                     (const uint4*)src, n16);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, uint64_t count, hipStream_t s) {
   if (count == 0) return hipSuccess;
   uint64_t blocks = (count + 255) / 256;

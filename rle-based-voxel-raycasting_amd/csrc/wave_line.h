@@ -115,6 +115,165 @@ __device__ inline int draw_span(int y0, int s2, float uzb, float ozb, float u2dz
   return drawn;
 }
 
+// Uniform DDA walk state of one line (Cuda_Render.h:270-367).
+struct DdaWalk {
+  float dd0, dd1, i0x, i0y, i1x, i1y;  // next crossing distances / positions
+  float g0x, g0y, g1x, g1y, gd0, gd1;  // their increments
+  float px, py, dnow;                  // last step taken
+  int idx;
+  int mip, gridx, gridz, mapswitch;
+  float dz, z, yswitch;
+  const uint2* map;
+  const uint16_t* slab;
+};
+
+// One chunk of up to 64 DDA steps: lane k holds step k and its gathered
+// pointer-map entry.  The mip level is constant within a chunk.
+struct DdaChunk {
+  int L, mip;
+  const uint16_t* slab;
+  float db, dn, pbx, pby;  // per lane: distances and position before the step
+  int ib;
+  uint2 ent;
+};
+
+// Advances the walk by one chunk and issues the pointer-map gather of its
+// cells.  Returns false when the line has reached the far distance.
+__device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& sc, int64_t zfar,
+                                      float vpx, float vpz, float fix_xf, float fix_yf, float* dl,
+                                      int lane) {
+  // distance mip switch (Cuda_Render.h:343-365), uniform
+  while (w.z > (float)w.mapswitch || w.yswitch > 512.0f) {
+    w.yswitch = w.yswitch * 0.5f;
+    if (w.mip < sc.nmips - 1) {
+      w.mip++;
+      w.gridx >>= 1;
+      w.gridz >>= 1;
+      w.map = sc.mip[w.mip].map;
+      w.slab = sc.mip[w.mip].slabs;
+    }
+    w.g0x *= 2;
+    w.g0y *= 2;
+    w.g1x *= 2;
+    w.g1y *= 2;
+    w.gd0 *= 2;
+    w.gd1 *= 2;
+    w.mapswitch *= 2;
+    w.dz *= 2;
+  }
+  // Iterations left before the next mip switch and before z passes z_far
+  // (z, dz and mapswitch are exact integers).
+  const int64_t zi = (int64_t)w.z, dzi = (int64_t)w.dz;
+  const int64_t n_mip = ((int64_t)w.mapswitch - zi) / dzi + 1;
+  const int64_t n_far = (zfar - zi) / dzi;
+  if (n_far <= 0) return false;
+  const int L = (int)min((int64_t)64, min(n_mip, n_far));
+  c.L = L;
+  c.mip = w.mip;
+  c.slab = w.slab;
+  c.db = c.dn = c.pbx = c.pby = 0;
+  c.ib = 0;
+  const SeqChunk cA = seq_chunk(w.dd0, w.gd0), cB = seq_chunk(w.dd1, w.gd1);
+  const SeqChunk cX0 = seq_chunk(w.i0x, w.g0x), cY0 = seq_chunk(w.i0y, w.g0y);
+  const SeqChunk cX1 = seq_chunk(w.i1x, w.g1x), cY1 = seq_chunk(w.i1y, w.g1y);
+  if (cA.ok && cB.ok && cX0.ok && cY0.ok && cX1.ok && cY1.ok) {
+    float* sA = dl;
+    float* sB = dl + 64;
+    float* sX0 = dl + 128;
+    float* sY0 = dl + 192;
+    float* sX1 = dl + 256;
+    float* sY1 = dl + 320;
+    int* slot = reinterpret_cast<int*>(dl + 384);
+    const float ak = seq_term(cA, lane), bk = seq_term(cB, lane);
+    sA[lane] = ak;
+    sB[lane] = bk;
+    sX0[lane] = seq_term(cX0, lane);
+    sY0[lane] = seq_term(cY0, lane);
+    sX1[lane] = seq_term(cX1, lane);
+    sY1[lane] = seq_term(cY1, lane);
+    __syncthreads();
+    // merge ranks: the sequences grow by about gd per term
+    const int gx = d_f2i(ceilf(__fdividef(ak - w.dd1, w.gd1)));
+    const int gz = d_f2i(ceilf(__fdividef(bk - w.dd0, w.gd0)));
+    const int rx = lane + rank_in<false>(sB, ak, gx);
+    const int rz = lane + rank_in<true>(sA, bk, gz);
+    if (rx < L) slot[rx] = lane;
+    if (rz < L) slot[rz] = 64 + lane;
+    const int nx = (int)__popcll(__ballot(rx < L));
+    const int nz = L - nx;
+    __syncthreads();
+    if (lane < L) {
+      const int cc = slot[lane];
+      c.dn = cc < 64 ? sA[cc] : sB[cc - 64];
+      if (lane == 0) {
+        c.db = w.dnow;
+        c.pbx = w.px;
+        c.pby = w.py;
+        c.ib = w.idx;
+      } else {
+        const int p = slot[lane - 1];
+        c.db = p < 64 ? sA[p] : sB[p - 64];
+        c.pbx = p < 64 ? sX0[p] : sX1[p - 64];
+        c.pby = p < 64 ? sY0[p] : sY1[p - 64];
+        c.ib = p < 64 ? 0 : 1;
+      }
+    }
+    const int cl = slot[L - 1];
+    w.dnow = cl < 64 ? sA[cl] : sB[cl - 64];
+    w.px = cl < 64 ? sX0[cl] : sX1[cl - 64];
+    w.py = cl < 64 ? sY0[cl] : sY1[cl - 64];
+    w.idx = cl < 64 ? 0 : 1;
+    w.dd0 = nx < 64 ? sA[nx] : sA[63] + w.gd0;
+    w.i0x = nx < 64 ? sX0[nx] : sX0[63] + w.g0x;
+    w.i0y = nx < 64 ? sY0[nx] : sY0[63] + w.g0y;
+    w.dd1 = nz < 64 ? sB[nz] : sB[63] + w.gd1;
+    w.i1x = nz < 64 ? sX1[nz] : sX1[63] + w.g1x;
+    w.i1y = nz < 64 ? sY1[nz] : sY1[63] + w.g1y;
+    __syncthreads();
+  } else {
+    // plain sequential walk (small distances, axis-aligned lines)
+    for (int k = 0; k < L; k++) {
+      const float db = w.dnow, pbx = w.px, pby = w.py;
+      const int ib = w.idx;
+      if (w.dd1 < w.dd0) {
+        w.dnow = w.dd1;
+        w.idx = 1;
+        w.dd1 += w.gd1;
+        w.px = w.i1x;
+        w.py = w.i1y;
+        w.i1x += w.g1x;
+        w.i1y += w.g1y;
+      } else {
+        w.dnow = w.dd0;
+        w.idx = 0;
+        w.px = w.i0x;
+        w.py = w.i0y;
+        w.dd0 += w.gd0;
+        w.i0x += w.g0x;
+        w.i0y += w.g0y;
+      }
+      if (k == lane) {
+        c.db = db;
+        c.dn = w.dnow;
+        c.pbx = pbx;
+        c.pby = pby;
+        c.ib = ib;
+      }
+    }
+  }
+  w.z += (float)L * w.dz;
+  // cell of each step (Cuda_Render.h:416-442) and its pointer-map entry
+  const int fix_x = (int)((float)(1 - c.ib) * fix_xf);
+  const int fix_z = (int)((float)c.ib * fix_yf);
+  const int voxel_x = d_f2i(vpx + c.pbx) + fix_x;
+  const int voxel_z = d_f2i(vpz + c.pby) + fix_z;
+  const int vx = (voxel_x >> w.mip) & (w.gridx - 1);
+  const int vz = (voxel_z >> w.mip) & (w.gridz - 1);
+  c.ent = make_uint2(0u, 0u);
+  if (lane < L) c.ent = w.map[vx + vz * w.gridx];
+  return true;
+}
+
 template <bool COUNT>
 __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_raymap& rm,
                                         uint32_t* __restrict__ row, int32_t* __restrict__ clip,
@@ -227,162 +386,52 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     sign_y = 1;
     frac_y = 1 - frac_y;
   }
-  float g0y = ddy / fabsf(ddx), g0x = sign_x;
-  float g1x = ddx / fabsf(ddy), g1y = sign_y;
-  float i0x = g0x * frac_x, i0y = g0y * frac_x;
-  float i1x = g1x * frac_y, i1y = g1y * frac_y;
-  float gd0 = sqrtf(g0x * g0x + g0y * g0y);
-  float gd1 = sqrtf(g1x * g1x + g1y * g1y);
-  float dd0 = sqrtf(i0x * i0x + i0y * i0y);
-  float dd1 = sqrtf(i1x * i1x + i1y * i1y);
-  float px = 0, py = 0, dnow = 0;
-  int idx = 0;
-
-  int mip = 0;
-  int gridx = sc.mip[0].sx, gridz = sc.mip[0].sz;
-  const uint2* map = sc.mip[0].map;
-  const uint16_t* slab = sc.mip[0].slabs;
+  DdaWalk wk;
+  wk.g0y = ddy / fabsf(ddx);
+  wk.g0x = sign_x;
+  wk.g1x = ddx / fabsf(ddy);
+  wk.g1y = sign_y;
+  wk.i0x = wk.g0x * frac_x;
+  wk.i0y = wk.g0y * frac_x;
+  wk.i1x = wk.g1x * frac_y;
+  wk.i1y = wk.g1y * frac_y;
+  wk.gd0 = sqrtf(wk.g0x * wk.g0x + wk.g0y * wk.g0y);
+  wk.gd1 = sqrtf(wk.g1x * wk.g1x + wk.g1y * wk.g1y);
+  wk.dd0 = sqrtf(wk.i0x * wk.i0x + wk.i0y * wk.i0y);
+  wk.dd1 = sqrtf(wk.i1x * wk.i1x + wk.i1y * wk.i1y);
+  wk.px = wk.py = wk.dnow = 0;
+  wk.idx = 0;
+  wk.mip = 0;
+  wk.gridx = sc.mip[0].sx;
+  wk.gridz = sc.mip[0].sz;
+  wk.map = sc.mip[0].map;
+  wk.slab = sc.mip[0].slabs;
+  wk.mapswitch = rm.mapswitch;
+  wk.dz = 1;
+  wk.z = 0;
+  wk.yswitch = vpy;
   const float z_add = sin_x;
   const float y_add = (vertical ? cos_x : 0.0f) * hmul;
   const int64_t zfar = rm.far_distance;
-  int mapswitch = rm.mapswitch;
-  float dz = 1, z = 0;
-  float yswitch = vpy;
   const float mountain = vpy;
 
-  for (;;) {
-    // ---- distance mip switch (Cuda_Render.h:343-365), uniform
-    while (z > (float)mapswitch || yswitch > 512.0f) {
-      yswitch = yswitch * 0.5f;
-      if (mip < sc.nmips - 1) {
-        mip++;
-        gridx >>= 1;
-        gridz >>= 1;
-        map = sc.mip[mip].map;
-        slab = sc.mip[mip].slabs;
-      }
-      g0x *= 2;
-      g0y *= 2;
-      g1x *= 2;
-      g1y *= 2;
-      gd0 *= 2;
-      gd1 *= 2;
-      mapswitch *= 2;
-      dz *= 2;
-    }
+  // Software pipeline: while chunk c is projected and replayed, the DDA of
+  // chunk c+1 runs and its pointer-map gather is in flight.  The walk does
+  // not depend on the replay (only the line's end does), so a chunk
+  // computed past the end of the line is simply dropped.
+  DdaChunk cur;
+  bool have = dda_next_chunk(wk, cur, sc, zfar, vpx, vpz, fix_xf, fix_yf, dl, lane);
+  while (have) {
     if (ymin >= ymax) break;
-    // Iterations left before the next mip switch and before z passes z_far
-    // (z, dz and mapswitch are exact integers).
-    const int64_t zi = (int64_t)z, dzi = (int64_t)dz;
-    const int64_t n_mip = ((int64_t)mapswitch - zi) / dzi + 1;
-    const int64_t n_far = (zfar - zi) / dzi;
-    if (n_far <= 0) break;
-    const int L = (int)min((int64_t)64, min(n_mip, n_far));
+    const int L = cur.L, mip = cur.mip;
+    const uint16_t* slab = cur.slab;
+    const uint2 ent = cur.ent;
 
-    // ---- 1. DDA chunk: lane k keeps step k
-    float my_db = 0, my_dn = 0, my_pbx = 0, my_pby = 0;
-    int my_ib = 0;
-    const SeqChunk cA = seq_chunk(dd0, gd0), cB = seq_chunk(dd1, gd1);
-    const SeqChunk cX0 = seq_chunk(i0x, g0x), cY0 = seq_chunk(i0y, g0y);
-    const SeqChunk cX1 = seq_chunk(i1x, g1x), cY1 = seq_chunk(i1y, g1y);
-    if (cA.ok && cB.ok && cX0.ok && cY0.ok && cX1.ok && cY1.ok) {
-      float* sA = dl;
-      float* sB = dl + 64;
-      float* sX0 = dl + 128;
-      float* sY0 = dl + 192;
-      float* sX1 = dl + 256;
-      float* sY1 = dl + 320;
-      int* slot = reinterpret_cast<int*>(dl + 384);
-      const float ak = seq_term(cA, lane), bk = seq_term(cB, lane);
-      sA[lane] = ak;
-      sB[lane] = bk;
-      sX0[lane] = seq_term(cX0, lane);
-      sY0[lane] = seq_term(cY0, lane);
-      sX1[lane] = seq_term(cX1, lane);
-      sY1[lane] = seq_term(cY1, lane);
-      __syncthreads();
-      // merge ranks: the sequences grow by about gd per term
-      const int gx = d_f2i(ceilf(__fdividef(ak - dd1, gd1)));
-      const int gz = d_f2i(ceilf(__fdividef(bk - dd0, gd0)));
-      const int rx = lane + rank_in<false>(sB, ak, gx);
-      const int rz = lane + rank_in<true>(sA, bk, gz);
-      if (rx < L) slot[rx] = lane;
-      if (rz < L) slot[rz] = 64 + lane;
-      const int nx = (int)__popcll(__ballot(rx < L));
-      const int nz = L - nx;
-      __syncthreads();
-      if (lane < L) {
-        const int c = slot[lane];
-        my_dn = c < 64 ? sA[c] : sB[c - 64];
-        if (lane == 0) {
-          my_db = dnow;
-          my_pbx = px;
-          my_pby = py;
-          my_ib = idx;
-        } else {
-          const int p = slot[lane - 1];
-          my_db = p < 64 ? sA[p] : sB[p - 64];
-          my_pbx = p < 64 ? sX0[p] : sX1[p - 64];
-          my_pby = p < 64 ? sY0[p] : sY1[p - 64];
-          my_ib = p < 64 ? 0 : 1;
-        }
-      }
-      const int cl = slot[L - 1];
-      dnow = cl < 64 ? sA[cl] : sB[cl - 64];
-      px = cl < 64 ? sX0[cl] : sX1[cl - 64];
-      py = cl < 64 ? sY0[cl] : sY1[cl - 64];
-      idx = cl < 64 ? 0 : 1;
-      dd0 = nx < 64 ? sA[nx] : sA[63] + gd0;
-      i0x = nx < 64 ? sX0[nx] : sX0[63] + g0x;
-      i0y = nx < 64 ? sY0[nx] : sY0[63] + g0y;
-      dd1 = nz < 64 ? sB[nz] : sB[63] + gd1;
-      i1x = nz < 64 ? sX1[nz] : sX1[63] + g1x;
-      i1y = nz < 64 ? sY1[nz] : sY1[63] + g1y;
-      __syncthreads();
-    } else {
-      // plain sequential walk (small distances, axis-aligned lines)
-      for (int k = 0; k < L; k++) {
-        const float db = dnow, pbx = px, pby = py;
-        const int ib = idx;
-        if (dd1 < dd0) {
-          dnow = dd1;
-          idx = 1;
-          dd1 += gd1;
-          px = i1x;
-          py = i1y;
-          i1x += g1x;
-          i1y += g1y;
-        } else {
-          dnow = dd0;
-          idx = 0;
-          px = i0x;
-          py = i0y;
-          dd0 += gd0;
-          i0x += g0x;
-          i0y += g0y;
-        }
-        if (k == lane) {
-          my_db = db;
-          my_dn = dnow;
-          my_pbx = pbx;
-          my_pby = pby;
-          my_ib = ib;
-        }
-      }
-    }
-    z += (float)L * dz;
-
-    // ---- 2./3. per-lane cell, gather and stateless projection
+    // ---- per-lane stateless projection (Cuda_Render.h:444-466)
     const bool valid = lane < L;
-    const int fix_x = (int)((float)(1 - my_ib) * fix_xf);
-    const int fix_z = (int)((float)my_ib * fix_yf);
-    const float ddelta = my_dn - my_db;
-    const float vs_x = ray_x * my_db;
-    const float vs_z = ray_z * my_db;
-    const int voxel_x = d_f2i(vpx + my_pbx) + fix_x;
-    const int voxel_z = d_f2i(vpz + my_pby) + fix_z;
-    const int vx = (voxel_x >> mip) & (gridx - 1);
-    const int vz = (voxel_z >> mip) & (gridz - 1);
+    const float ddelta = cur.dn - cur.db;
+    const float vs_x = ray_x * cur.db;
+    const float vs_z = ray_z * cur.db;
     const float corr_x = ray_x * ddelta;
     const float corr_z = ray_z * ddelta;
     const float pz = cos_x * vs_z + sin_x * mountain;
@@ -392,8 +441,6 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     float cyy = vertical ? -sin_x * corr_z : corr_x;
     cyy *= hmul;
     const float top = pz * half + py3;
-    uint2 ent = make_uint2(0u, 0u);
-    if (valid) ent = map[vx + vz * gridx];
     const uint32_t slen = ent.y & 0xffffu;
     const uint16_t* runs = slab + 2 + ent.x;
 
@@ -437,19 +484,30 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
       }
     }
 
-    // ---- 4. run cache and per-candidate precompute.  The horizon only
-    // rises, so F > ymin now is necessary for acting; the clip-top test is
-    // monotone in ymin only for pz > 0.
+    // ---- run cache: the head of each candidate column, fetched now and
+    // stored to LDS after the next chunk's DDA (its latency overlaps that
+    // work).  The horizon only rises, so F > ymin now is necessary for
+    // acting; the clip-top test is monotone in ymin only for pz > 0.
     const bool cand = valid && F > ymin && !(pz > 0.0f && top <= pz * (float)ymin);
     const uint32_t rc_base = ent.x & ~7u;  // 16-byte aligned start
     const int rc_delta = (int)(ent.x - rc_base);
     const uint16_t* cached = rc + lane * kRunCache + rc_delta;
     const int ccap = cand ? kRunCache - rc_delta : 0;
+    uint4 rcv[kRunCache / 8];
     if (cand) {
       const uint4* src = reinterpret_cast<const uint4*>(slab + rc_base);
+#pragma unroll
+      for (int i = 0; i < kRunCache / 8; i++) rcv[i] = src[i];
+    }
+
+    // ---- next chunk: DDA and pointer gather
+    DdaChunk nxt;
+    const bool have_next = dda_next_chunk(wk, nxt, sc, zfar, vpx, vpz, fix_xf, fix_yf, dl, lane);
+
+    if (cand) {
       uint4* dst = reinterpret_cast<uint4*>(rc + lane * kRunCache);
 #pragma unroll
-      for (int i = 0; i < kRunCache / 8; i++) dst[i] = src[i];
+      for (int i = 0; i < kRunCache / 8; i++) dst[i] = rcv[i];
     }
     // first element's lower end and span parameters (Cuda_Render.h:554-660)
     int s1F = 0;
@@ -506,8 +564,8 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
       }
     }
 
-    // ---- 5. ordered replay of the steps that act
-    int cur = 0;
+    // ---- ordered replay of the steps that act
+    int cur_step = 0;
     bool done = false;
     for (;;) {
       if (ymin >= ymax) {
@@ -515,13 +573,13 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
         break;
       }
       const bool culled = top <= pz * (float)ymin;
-      const bool act = valid && lane >= cur && !culled && F > ymin;
+      const bool act = valid && lane >= cur_step && !culled && F > ymin;
       const uint64_t am = __ballot(act);
       const int t = am ? (int)__builtin_ctzll(am) : L;
       if (COUNT) {
-        // no-op steps [cur, t): pointer loads and elements walked to the break
-        const bool noop_loaded = valid && lane >= cur && lane < t && !culled;
-        cnt.steps += (uint64_t)(t - cur);
+        // no-op steps [cur_step, t): pointer loads and elements walked to the break
+        const bool noop_loaded = valid && lane >= cur_step && lane < t && !culled;
+        cnt.steps += (uint64_t)(t - cur_step);
         cnt.ptr += (uint64_t)__popcll(__ballot(noop_loaded));
         unsigned long long e = noop_loaded ? nel : 0, r = (noop_loaded && nel) ? nel - 1 : 0;
         for (int o = 32; o > 0; o >>= 1) {
@@ -586,7 +644,7 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
         }
         // the next element tested breaks unless it reaches above ymin
         if (readlane_i(F2, t) <= ymin) {
-          cur = t + 1;
+          cur_step = t + 1;
           continue;
         }
         k0 = (uint32_t)readlane_i(kF, t) + 1;
@@ -667,9 +725,11 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
                                 tcap, tcol, 2 + (int)tslen, row, bits, lane);
         if (COUNT) cnt.col += (uint64_t)d;
       }
-      cur = t + 1;
+      cur_step = t + 1;
     }
     if (done) break;
+    cur = nxt;
+    have = have_next;
   }
   if (COUNT && lane == 0) {
     atomicAdd(counts + 0, 1ull);

@@ -162,6 +162,7 @@ _sigs = {
     "rlevox_dev_alloc": (C.c_int, [_vp, C.c_uint64, P(_vp)]),
     "rlevox_dev_free": (C.c_int, [_vp, _vp]),
     "rlevox_dev_fill_u32": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint64, _vp]),
+    "rlevox_dev_copy": (C.c_int, [_vp, _vp, _vp, C.c_uint64, _vp]),
     "rlevox_memcpy_htod": (C.c_int, [_vp, _vp, _vp, C.c_uint64]),
     "rlevox_memcpy_dtoh": (C.c_int, [_vp, _vp, _vp, C.c_uint64]),
     "rlevox_stream_create": (C.c_int, [_vp, P(_vp)]),
@@ -331,6 +332,10 @@ class DeviceBuffer:
     def fill_u32(self, value, count=None, stream=None):
         n = self.nbytes // 4 if count is None else count
         check(lib.rlevox_dev_fill_u32(self.ctx.handle, self.ptr, value, n, stream))
+
+    def copy_from(self, other, nbytes=None, stream=None):
+        n = self.nbytes if nbytes is None else int(nbytes)
+        check(lib.rlevox_dev_copy(self.ctx.handle, self.ptr, other.ptr, n, stream))
 
     def free(self):
         if self.ptr:
