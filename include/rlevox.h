@@ -37,6 +37,8 @@ extern "C" {
 /* Framebuffer value of a pixel that shows no voxel. */
 #define RLEVOX_SKY_COLOUR 0x8844u
 #define RLEVOX_SKY_DEPTH 0x7fffffff
+/* Largest offset_shift of a mip level (2^36 slab words = 128 GiB). */
+#define RLEVOX_MAX_OFFSET_SHIFT 4
 /* Largest render size (screen width) accepted. */
 #define RLEVOX_MAX_RS 8192
 
@@ -62,9 +64,15 @@ typedef struct rlevox_hscene rlevox_hscene; /* a scene in host memory */
 typedef struct rlevox_server rlevox_server; /* continuous-batching request loop */
 
 /* One mip level (Map4, Rle4.h:7-21).  ptrmap may be NULL on upload: it is
- * then rebuilt from the slabs as RLE4::load does (Rle4.cpp:295-314). */
+ * then rebuilt from the slabs as RLE4::load does (Rle4.cpp:295-314).
+ * offset_shift widens the 32-bit slab offsets: each column starts at a
+ * multiple of 2^offset_shift slab words and its pointer-map entry holds the
+ * word offset >> offset_shift (0 = the reference's layout, 32-bit offsets in
+ * uint16 units; 1..4 let one level hold up to 2^(32+shift) words; padding
+ * words between columns are never read). */
 typedef struct {
-  int32_t sx, sy, sz, reserved;
+  int32_t sx, sy, sz;
+  int32_t offset_shift;
   const uint32_t* ptrmap; /* 2*sx*sz words */
   const uint16_t* slabs;  /* nslabs words */
   uint64_t nslabs;
@@ -194,9 +202,18 @@ rlevox_status rlevox_raymap_compute(const rlevox_pose* pose, int32_t W, int32_t 
                                     const rlevox_render_params* params,
                                     rlevox_raymap* out);
 
-/* Pointer-map rebuild of RLE4::load (Rle4.cpp:295-314). */
+/* Pointer-map rebuild of RLE4::load (Rle4.cpp:295-314); with offset_shift > 0
+ * each column starts at the next multiple of 2^offset_shift words. */
 rlevox_status rlevox_build_ptrmap(const uint16_t* slabs, uint64_t nslabs, int32_t sx,
-                                  int32_t sz, uint32_t* ptrmap_out);
+                                  int32_t sz, int32_t offset_shift, uint32_t* ptrmap_out);
+
+/* Re-packs a scene with columns aligned to 2^offset_shift slab words (the
+ * decoded voxels are unchanged).  offset_shift >= 3 puts every column header
+ * on a 16-byte boundary, which the raycast's run cache loads whole; larger
+ * shifts also lift the 2^32-word limit of a level.  No counterpart in the
+ * reference (its layout is offset_shift 0). */
+rlevox_status rlevox_repack(const rlevox_mip* mips, int32_t nmips, int32_t offset_shift,
+                            rlevox_hscene** out);
 
 /* Checks that every column is well formed and stays inside its slab array. */
 rlevox_status rlevox_scene_validate(const rlevox_mip* mips, int32_t nmips);
@@ -209,7 +226,8 @@ rlevox_status rlevox_encode_dense(int32_t sx, int32_t sy, int32_t sz, const uint
                                   const uint16_t* colour, int32_t nmips,
                                   rlevox_hscene** out);
 
-/* .rle4 files (RLE4::save / RLE4::load, Rle4.cpp:220-384). */
+/* .rle4 files (RLE4::save / RLE4::load, Rle4.cpp:220-384).  The format has
+ * no field for offset_shift: only offset_shift 0 levels are saved. */
 rlevox_status rlevox_rle4_save(const char* path, const rlevox_mip* mips, int32_t nmips);
 rlevox_status rlevox_rle4_load(const char* path, rlevox_hscene** out);
 

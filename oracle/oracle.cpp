@@ -522,6 +522,7 @@ static void orc_render_line(int x, const OrcScene& sc, const rlevox_raymap& rm, 
   pos3d_y_add *= res_x2_mul_reverse;
   const uint32_t* map_ptr = sc.mips[0].ptrmap;
   const uint16_t* slab_ptr = sc.mips[0].slabs;
+  int offset_shift = sc.mips[0].offset_shift;  // widened offsets (0 = reference layout)
   float z = 0;
   float y_map_switch = vpy;
 
@@ -537,6 +538,7 @@ static void orc_render_line(int x, const OrcScene& sc, const rlevox_raymap& rm, 
         gridz >>= 1;
         map_ptr = sc.mips[mip_lvl].ptrmap;
         slab_ptr = sc.mips[mip_lvl].slabs;
+        offset_shift = sc.mips[mip_lvl].offset_shift;
       }
       grad0_x *= 2;
       grad0_y *= 2;
@@ -599,7 +601,7 @@ static void orc_render_line(int x, const OrcScene& sc, const rlevox_raymap& rm, 
     // Pointer-map gather (Cuda_Render.h:474-478)
     const uint32_t* ent = map_ptr + 2 * ((size_t)vx + (size_t)vz * (size_t)gridx);
     c_ptr++;
-    const uint32_t slab_offset = ent[0];
+    const uint64_t slab_offset = (uint64_t)ent[0] << offset_shift;
     const uint32_t len_first = ent[1];
     const uint16_t slen = (uint16_t)len_first;
     float corr_zz = cos_x * correct_z;
@@ -1034,7 +1036,7 @@ int orc_decode(const rlevox_mip* mip, uint8_t* surf, uint16_t* col) {
   for (int k = 0; k < sz; k++)
     for (int i = 0; i < sx; i++) {
       const size_t c = i + (size_t)k * sx;
-      const uint64_t ofs = mip->ptrmap[2 * c];
+      const uint64_t ofs = (uint64_t)mip->ptrmap[2 * c] << mip->offset_shift;
       if (ofs + 2 > mip->nslabs) return 1;
       const uint16_t* p = mip->slabs + ofs;
       const int count = p[0], ncol = p[1];

@@ -97,7 +97,8 @@ rlevox_status rlevox_scene_upload(rlevox_ctx* ctx, const rlevox_mip* mips, int32
   for (int i = 0; i < (int)m.size(); i++) {
     if (!m[i].ptrmap && m[i].slabs && m[i].sx > 0 && m[i].sz > 0) {
       rebuilt[i].resize((size_t)2 * m[i].sx * m[i].sz);
-      st = rlevox_build_ptrmap(m[i].slabs, m[i].nslabs, m[i].sx, m[i].sz, rebuilt[i].data());
+      st = rlevox_build_ptrmap(m[i].slabs, m[i].nslabs, m[i].sx, m[i].sz, m[i].offset_shift,
+                               rebuilt[i].data());
       if (st != RLEVOX_OK) return st;
       m[i].ptrmap = rebuilt[i].data();
     }
@@ -129,6 +130,7 @@ rlevox_status rlevox_scene_upload(rlevox_ctx* ctx, const rlevox_mip* mips, int32
     s->dev.mip[i].slabs = (const uint16_t*)ds;
     s->dev.mip[i].sx = m[i].sx;
     s->dev.mip[i].sz = m[i].sz;
+    s->dev.mip[i].shift = m[i].offset_shift;
     s->sy[i] = m[i].sy;
     s->bytes += map_bytes + slab_bytes;
   }

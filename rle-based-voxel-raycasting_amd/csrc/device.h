@@ -20,6 +20,8 @@ struct MipDev {
   const uint2* map;
   const uint16_t* slabs;
   int32_t sx, sz;
+  int32_t shift;  // column word offset = map[c].x << shift
+  int32_t pad;
 };
 
 struct SceneDev {
@@ -211,6 +213,7 @@ __device__ inline void render_line(int x, const SceneDev& sc, const rlevox_rayma
   int gridx = sc.mip[0].sx, gridz = sc.mip[0].sz;
   const uint2* map = sc.mip[0].map;
   const uint16_t* slab = sc.mip[0].slabs;
+  int shift = sc.mip[0].shift;
   const float z_add = sin_x;
   const float y_add = (vertical ? cos_x : 0.0f) * hmul;
   const float zfar = (float)rm.far_distance;
@@ -229,6 +232,7 @@ __device__ inline void render_line(int x, const SceneDev& sc, const rlevox_rayma
         gridz >>= 1;
         map = sc.mip[mip].map;
         slab = sc.mip[mip].slabs;
+        shift = sc.mip[mip].shift;
       }
       g0x *= 2;
       g0y *= 2;
@@ -290,7 +294,7 @@ __device__ inline void render_line(int x, const SceneDev& sc, const rlevox_rayma
     const float czz = cos_x * corr_z;
     float cyy = vertical ? -sin_x * corr_z : corr_x;
     cyy *= hmul;
-    const uint16_t* runs = slab + 2 + ent.x;
+    const uint16_t* runs = slab + 2 + ((uint64_t)ent.x << shift);
     const uint16_t* colours = runs + slen;
     uint32_t w = ent.y >> 16;
     int ygen = 0;

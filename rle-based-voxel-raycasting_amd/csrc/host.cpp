@@ -52,24 +52,27 @@ void parallel_for(int64_t n, int threads, int64_t chunk, const std::function<voi
 }
 
 // Pointer-map rebuild of RLE4::load (Rle4.cpp:295-314): columns are walked x
-// fastest; each entry is [offset, count | first<<16] where `first` is the
-// word after the column header (the cached first element).
-static rlevox_status build_ptrmap(const uint16_t* slabs, uint64_t nslabs, int sx, int sz,
-                                  uint32_t* out) {
+// fastest; each entry is [offset >> shift, count | first<<16] where `first`
+// is the word after the column header (the cached first element).  With a
+// shift, each column starts at the next multiple of 2^shift words.
+rlevox_status build_ptrmap(const uint16_t* slabs, uint64_t nslabs, int sx, int sz, int shift,
+                           uint32_t* out) {
   uint64_t ofs = 0;
   const uint64_t ncols = (uint64_t)sx * (uint64_t)sz;
+  const uint64_t align = (1ull << shift) - 1;
   for (uint64_t c = 0; c < ncols; c++) {
     if (ofs + 2 > nslabs)
       return fail(RLEVOX_ERR_INVALID_SCENE, "ptrmap: slabs end inside the header of column %llu",
                   (unsigned long long)c);
-    if (ofs > 0xffffffffull)
-      return fail(RLEVOX_ERR_INVALID_SCENE, "ptrmap: offset of column %llu exceeds 32 bits",
-                  (unsigned long long)c);
+    if ((ofs >> shift) > 0xffffffffull)
+      return fail(RLEVOX_ERR_INVALID_SCENE, "ptrmap: offset of column %llu exceeds 32+%d bits",
+                  (unsigned long long)c, shift);
     const uint32_t count = slabs[ofs];
     const uint32_t first = (ofs + 2 < nslabs) ? slabs[ofs + 2] : 0;
-    out[2 * c] = (uint32_t)ofs;
+    out[2 * c] = (uint32_t)(ofs >> shift);
     out[2 * c + 1] = count + (first << 16);
     ofs += (uint64_t)slabs[ofs] + slabs[ofs + 1] + 2;
+    if (c + 1 < ncols) ofs = (ofs + align) & ~align;
   }
   if (ofs > nslabs)
     return fail(RLEVOX_ERR_INVALID_SCENE, "ptrmap: last column runs past the slab array");
@@ -97,6 +100,9 @@ static rlevox_status validate(const rlevox_mip* mips, int nmips) {
                   std::max(1, mips[0].sy >> m));
     if (!mp.slabs || !mp.ptrmap)
       return fail(RLEVOX_ERR_INVALID_ARG, "scene: mip %d has a null array", m);
+    if (mp.offset_shift < 0 || mp.offset_shift > RLEVOX_MAX_OFFSET_SHIFT)
+      return fail(RLEVOX_ERR_INVALID_SCENE, "scene: mip %d offset_shift %d outside 0..%d", m,
+                  mp.offset_shift, RLEVOX_MAX_OFFSET_SHIFT);
     const int64_t ncols = (int64_t)mp.sx * mp.sz;
     std::atomic<int64_t> bad(-1);
     std::atomic<int> why(0);
@@ -104,7 +110,7 @@ static rlevox_status validate(const rlevox_mip* mips, int nmips) {
       if (bad.load(std::memory_order_relaxed) >= 0) return;
       for (int64_t x = 0; x < mp.sx; x++) {
         const int64_t c = z * mp.sx + x;
-        const uint64_t ofs = mp.ptrmap[2 * c];
+        const uint64_t ofs = (uint64_t)mp.ptrmap[2 * c] << mp.offset_shift;
         const uint32_t lf = mp.ptrmap[2 * c + 1];
         int reason = 0;
         if (ofs + 2 > mp.nslabs) reason = 1;
@@ -155,11 +161,12 @@ const char* rlevox_last_error(void) { return t_error.c_str(); }
 int32_t rlevox_abi_version(void) { return RLEVOX_ABI_VERSION; }
 
 rlevox_status rlevox_build_ptrmap(const uint16_t* slabs, uint64_t nslabs, int32_t sx, int32_t sz,
-                                  uint32_t* ptrmap_out) {
+                                  int32_t offset_shift, uint32_t* ptrmap_out) {
   clear_error();
-  if (!slabs || !ptrmap_out || sx < 1 || sz < 1)
+  if (!slabs || !ptrmap_out || sx < 1 || sz < 1 || offset_shift < 0 ||
+      offset_shift > RLEVOX_MAX_OFFSET_SHIFT)
     return fail(RLEVOX_ERR_INVALID_ARG, "build_ptrmap: bad argument");
-  return build_ptrmap(slabs, nslabs, sx, sz, ptrmap_out);
+  return build_ptrmap(slabs, nslabs, sx, sz, offset_shift, ptrmap_out);
 }
 
 rlevox_status rlevox_scene_validate(const rlevox_mip* mips, int32_t nmips) {
@@ -177,7 +184,7 @@ rlevox_status rlevox_hscene_mip(const rlevox_hscene* s, int32_t m, rlevox_mip* o
   out->sx = l.sx;
   out->sy = l.sy;
   out->sz = l.sz;
-  out->reserved = 0;
+  out->offset_shift = l.shift;
   out->ptrmap = l.ptrmap.data();
   out->slabs = l.slabs.data();
   out->nslabs = l.slabs.size();
@@ -192,6 +199,10 @@ rlevox_status rlevox_rle4_save(const char* path, const rlevox_mip* mips, int32_t
   clear_error();
   if (!path || !mips || nmips < 1 || nmips > RLEVOX_MAX_MIPS)
     return fail(RLEVOX_ERR_INVALID_ARG, "rle4_save: bad argument");
+  for (int m = 0; m < nmips; m++)
+    if (mips[m].offset_shift != 0)
+      return fail(RLEVOX_ERR_INVALID_ARG, "rle4_save: mip %d has offset_shift %d; the format "
+                  "stores contiguous columns only", m, mips[m].offset_shift);
   for (int m = 0; m < nmips; m++)
     if (mips[m].nslabs > 0x7fffffffull)
       return fail(RLEVOX_ERR_INVALID_ARG, "rle4_save: mip %d has %llu slab words; the format holds "
@@ -238,7 +249,7 @@ rlevox_status rlevox_rle4_load(const char* path, rlevox_hscene** out) {
       break;
     }
     l.ptrmap.resize((size_t)2 * l.sx * l.sz);
-    st = build_ptrmap(l.slabs.data(), l.slabs.size(), l.sx, l.sz, l.ptrmap.data());
+    st = build_ptrmap(l.slabs.data(), l.slabs.size(), l.sx, l.sz, 0, l.ptrmap.data());
     s->nmips = m + 1;
   }
   std::fclose(f);

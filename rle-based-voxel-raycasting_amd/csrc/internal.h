@@ -19,8 +19,12 @@ void clear_error();
 // concurrency), in chunks handed out dynamically.
 void parallel_for(int64_t n, int threads, int64_t chunk, const std::function<void(int64_t)>& body);
 
+rlevox_status build_ptrmap(const uint16_t* slabs, uint64_t nslabs, int sx, int sz, int shift,
+                           uint32_t* out);
+
 struct HostLevel {
   int32_t sx = 0, sy = 0, sz = 0;
+  int32_t shift = 0;  // offset_shift of the level
   std::vector<uint32_t> ptrmap;
   std::vector<uint16_t> slabs;
 };

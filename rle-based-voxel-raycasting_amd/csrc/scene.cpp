@@ -192,20 +192,30 @@ rlevox_status encode_level(const Level& L, ColourFn&& colour, HostLevel& out, in
       if (nel > 65535 || ncol > 65535) size[z * L.sx + x] = 0xffffffffu;
     }
   });
-  std::vector<uint64_t> start(ncols + 1);
-  start[0] = 0;
-  for (int64_t c = 0; c < ncols; c++) {
+  for (int64_t c = 0; c < ncols; c++)
     if (size[c] == 0xffffffffu)
       return fail(RLEVOX_ERR_INVALID_SCENE, "encode: column %lld has more than 65535 runs or colours",
                   (long long)c);
-    start[c + 1] = start[c] + size[c];
+  // Column starts: contiguous (the reference's layout) while every offset
+  // fits 32 bits, else aligned to the smallest 2^shift that makes them fit.
+  std::vector<uint64_t> start(ncols + 1);
+  int shift = 0;
+  for (;; shift++) {
+    if (shift > RLEVOX_MAX_OFFSET_SHIFT)
+      return fail(RLEVOX_ERR_INVALID_SCENE, "encode: level too large for 32+%d-bit offsets",
+                  RLEVOX_MAX_OFFSET_SHIFT);
+    const uint64_t align = (1ull << shift) - 1;
+    start[0] = 0;
+    for (int64_t c = 0; c < ncols; c++) {
+      start[c + 1] = start[c] + size[c];
+      if (c + 1 < ncols) start[c + 1] = (start[c + 1] + align) & ~align;
+    }
+    if ((start[ncols - 1] >> shift) <= 0xffffffffull) break;
   }
-  if (start[ncols] > 0xffffffffull + 2)
-    return fail(RLEVOX_ERR_INVALID_SCENE, "encode: level needs %llu slab words; offsets are 32-bit",
-                (unsigned long long)start[ncols]);
   out.sx = L.sx;
   out.sy = L.sy;
   out.sz = L.sz;
+  out.shift = shift;
   out.slabs.assign(start[ncols], 0);
   out.ptrmap.assign((size_t)2 * ncols, 0);
   parallel_for(L.sz, threads, 4, [&](int64_t z) {
@@ -237,7 +247,7 @@ rlevox_status encode_level(const Level& L, ColourFn&& colour, HostLevel& out, in
   parallel_for(ncols, threads, 65536, [&](int64_t c) {
     const uint64_t ofs = start[c];
     const uint32_t first = (ofs + 2 < total) ? out.slabs[ofs + 2] : 0;
-    out.ptrmap[2 * c] = (uint32_t)ofs;
+    out.ptrmap[2 * c] = (uint32_t)(ofs >> shift);
     out.ptrmap[2 * c + 1] = (uint32_t)out.slabs[ofs] + (first << 16);
   });
   return RLEVOX_OK;
@@ -425,6 +435,49 @@ extern "C" rlevox_status rlevox_encode_dense(int32_t sx, int32_t sy, int32_t sz,
     rlevox_status st = encode_level(
         cur, [&](int x, int y, int z) { return col[x + y * lsx + z * lsxy]; }, s->lv[m], 0);
     if (st != RLEVOX_OK) return st;
+    s->nmips = m + 1;
+  }
+  *out = s.release();
+  return RLEVOX_OK;
+}
+
+// Re-packs levels with columns aligned to 2^offset_shift words.  Column
+// contents are copied unchanged, so the decoded scene is the same.
+extern "C" rlevox_status rlevox_repack(const rlevox_mip* mips, int32_t nmips, int32_t offset_shift,
+                                       rlevox_hscene** out) {
+  clear_error();
+  if (!mips || !out || offset_shift < 0 || offset_shift > RLEVOX_MAX_OFFSET_SHIFT)
+    return fail(RLEVOX_ERR_INVALID_ARG, "repack: bad argument");
+  *out = nullptr;
+  rlevox_status st = rlevox_scene_validate(mips, nmips);
+  if (st != RLEVOX_OK) return st;
+  std::unique_ptr<rlevox_hscene> s(new rlevox_hscene());
+  for (int m = 0; m < nmips; m++) {
+    const rlevox_mip& in = mips[m];
+    const int64_t ncols = (int64_t)in.sx * in.sz;
+    const uint64_t align = (1ull << offset_shift) - 1;
+    std::vector<uint64_t> start(ncols + 1);
+    start[0] = 0;
+    for (int64_t c = 0; c < ncols; c++) {
+      const uint16_t* p = in.slabs + ((uint64_t)in.ptrmap[2 * c] << in.offset_shift);
+      start[c + 1] = start[c] + 2 + p[0] + p[1];
+      if (c + 1 < ncols) start[c + 1] = (start[c + 1] + align) & ~align;
+    }
+    if (ncols > 0 && (start[ncols - 1] >> offset_shift) > 0xffffffffull)
+      return fail(RLEVOX_ERR_INVALID_ARG, "repack: mip %d needs a larger offset_shift", m);
+    HostLevel& l = s->lv[m];
+    l.sx = in.sx;
+    l.sy = in.sy;
+    l.sz = in.sz;
+    l.shift = offset_shift;
+    l.slabs.assign(start[ncols], 0);
+    l.ptrmap.assign((size_t)2 * ncols, 0);
+    parallel_for(ncols, 0, 65536, [&](int64_t c) {
+      const uint16_t* p = in.slabs + ((uint64_t)in.ptrmap[2 * c] << in.offset_shift);
+      std::memcpy(l.slabs.data() + start[c], p, (size_t)(2 + p[0] + p[1]) * 2);
+      l.ptrmap[2 * c] = (uint32_t)(start[c] >> offset_shift);
+      l.ptrmap[2 * c + 1] = in.ptrmap[2 * c + 1];
+    });
     s->nmips = m + 1;
   }
   *out = s.release();

@@ -121,7 +121,7 @@ struct DdaWalk {
   float g0x, g0y, g1x, g1y, gd0, gd1;  // their increments
   float px, py, dnow;                  // last step taken
   int idx;
-  int mip, gridx, gridz, mapswitch;
+  int mip, gridx, gridz, mapswitch, shift;
   float dz, z, yswitch;
   const uint2* map;
   const uint16_t* slab;
@@ -130,7 +130,7 @@ struct DdaWalk {
 // One chunk of up to 64 DDA steps: lane k holds step k and its gathered
 // pointer-map entry.  The mip level is constant within a chunk.
 struct DdaChunk {
-  int L, mip;
+  int L, mip, shift;
   const uint16_t* slab;
   float db, dn, pbx, pby;  // per lane: distances and position before the step
   int ib;
@@ -151,6 +151,7 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
       w.gridz >>= 1;
       w.map = sc.mip[w.mip].map;
       w.slab = sc.mip[w.mip].slabs;
+      w.shift = sc.mip[w.mip].shift;
     }
     w.g0x *= 2;
     w.g0y *= 2;
@@ -170,6 +171,7 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
   const int L = (int)min((int64_t)64, min(n_mip, n_far));
   c.L = L;
   c.mip = w.mip;
+  c.shift = w.shift;
   c.slab = w.slab;
   c.db = c.dn = c.pbx = c.pby = 0;
   c.ib = 0;
@@ -406,6 +408,7 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
   wk.gridz = sc.mip[0].sz;
   wk.map = sc.mip[0].map;
   wk.slab = sc.mip[0].slabs;
+  wk.shift = sc.mip[0].shift;
   wk.mapswitch = rm.mapswitch;
   wk.dz = 1;
   wk.z = 0;
@@ -423,9 +426,10 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
   bool have = dda_next_chunk(wk, cur, sc, zfar, vpx, vpz, fix_xf, fix_yf, dl, lane);
   while (have) {
     if (ymin >= ymax) break;
-    const int L = cur.L, mip = cur.mip;
+    const int L = cur.L, mip = cur.mip, shift = cur.shift;
     const uint16_t* slab = cur.slab;
     const uint2 ent = cur.ent;
+    const uint64_t wofs = (uint64_t)ent.x << shift;  // column's slab word offset
 
     // ---- per-lane stateless projection (Cuda_Render.h:444-466)
     const bool valid = lane < L;
@@ -442,7 +446,7 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     cyy *= hmul;
     const float top = pz * half + py3;
     const uint32_t slen = ent.y & 0xffffu;
-    const uint16_t* runs = slab + 2 + ent.x;
+    const uint16_t* runs = slab + 2 + wofs;
 
     // F: projected top of the first drawn element in front of the eye.
     int F = INT32_MIN, kF = -1, ygF = 0;
@@ -489,8 +493,8 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     // work).  The horizon only rises, so F > ymin now is necessary for
     // acting; the clip-top test is monotone in ymin only for pz > 0.
     const bool cand = valid && F > ymin && !(pz > 0.0f && top <= pz * (float)ymin);
-    const uint32_t rc_base = ent.x & ~7u;  // 16-byte aligned start
-    const int rc_delta = (int)(ent.x - rc_base);
+    const uint64_t rc_base = wofs & ~(uint64_t)7;  // 16-byte aligned start
+    const int rc_delta = (int)(wofs - rc_base);
     const uint16_t* cached = rc + lane * kRunCache + rc_delta;
     const int ccap = cand ? kRunCache - rc_delta : 0;
     uint4 rcv[kRunCache / 8];
@@ -595,13 +599,13 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
         cnt.ptr++;
       }
 
-      const uint32_t tofs = (uint32_t)readlane_i((int)ent.x, t);
+      const uint64_t tofs = (uint64_t)(uint32_t)readlane_i((int)ent.x, t) << shift;
       const uint32_t tlf = (uint32_t)readlane_i((int)ent.y, t);
       const uint32_t tslen = tlf & 0xffffu;
       const uint16_t* truns = slab + 2 + tofs;
       const uint16_t* tcol = truns + tslen;
       const bool tcached = readlane_i((int)cand, t) != 0;
-      const int tdelta = (int)(tofs - (tofs & ~7u));
+      const int tdelta = (int)(tofs & 7u);
       const uint16_t* tcache = rc + t * kRunCache + tdelta;
       const int tcap = tcached ? kRunCache - tdelta : 0;
       const float tpz = readlane_f(pz, t), tpy = readlane_f(py3, t);

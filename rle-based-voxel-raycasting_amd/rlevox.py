@@ -34,7 +34,7 @@ class RlevoxError(RuntimeError):
 
 
 class Mip(C.Structure):
-    _fields_ = [("sx", C.c_int32), ("sy", C.c_int32), ("sz", C.c_int32), ("reserved", C.c_int32),
+    _fields_ = [("sx", C.c_int32), ("sy", C.c_int32), ("sz", C.c_int32), ("offset_shift", C.c_int32),
                 ("ptrmap", C.c_void_p), ("slabs", C.c_void_p), ("nslabs", C.c_uint64)]
 
 
@@ -128,7 +128,8 @@ _sigs = {
     "rlevox_last_error": (C.c_char_p, []),
     "rlevox_abi_version": (C.c_int32, []),
     "rlevox_raymap_compute": (C.c_int, [P(Pose), C.c_int32, C.c_int32, P(RenderParams), P(RayMap)]),
-    "rlevox_build_ptrmap": (C.c_int, [_vp, C.c_uint64, C.c_int32, C.c_int32, _vp]),
+    "rlevox_build_ptrmap": (C.c_int, [_vp, C.c_uint64, C.c_int32, C.c_int32, C.c_int32, _vp]),
+    "rlevox_repack": (C.c_int, [P(Mip), C.c_int32, C.c_int32, P(_vp)]),
     "rlevox_scene_validate": (C.c_int, [P(Mip), C.c_int32]),
     "rlevox_gen_scene": (C.c_int, [P(GenParams), P(_vp)]),
     "rlevox_encode_dense": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _vp, _vp, C.c_int32, P(_vp)]),
@@ -237,7 +238,8 @@ class HostScene:
             check(lib.rlevox_hscene_mip(self.handle, m, C.byref(mp)))
             ptr = _owned_array(self._native, mp.ptrmap, C.c_uint32, np.uint32, 2 * mp.sx * mp.sz)
             slabs = _owned_array(self._native, mp.slabs, C.c_uint16, np.uint16, mp.nslabs)
-            self.mips.append({"sx": mp.sx, "sy": mp.sy, "sz": mp.sz, "ptrmap": ptr, "slabs": slabs})
+            self.mips.append({"sx": mp.sx, "sy": mp.sy, "sz": mp.sz, "ptrmap": ptr, "slabs": slabs,
+                              "shift": mp.offset_shift})
 
     def mip_array(self):
         return mip_array(self.mips)
@@ -258,6 +260,7 @@ def mip_array(mips):
     arr = (Mip * len(mips))()
     for i, m in enumerate(mips):
         arr[i].sx, arr[i].sy, arr[i].sz = m["sx"], m["sy"], m["sz"]
+        arr[i].offset_shift = m.get("shift", 0)
         arr[i].ptrmap = m["ptrmap"].ctypes.data if m.get("ptrmap") is not None else None
         arr[i].slabs = m["slabs"].ctypes.data
         arr[i].nslabs = m["slabs"].size
@@ -296,10 +299,18 @@ def rle4_load(path):
     return HostScene(h.value)
 
 
-def build_ptrmap(slabs, sx, sz):
+def build_ptrmap(slabs, sx, sz, shift=0):
     out = np.zeros(2 * sx * sz, np.uint32)
-    check(lib.rlevox_build_ptrmap(_ptr(slabs), slabs.size, sx, sz, _ptr(out)))
+    check(lib.rlevox_build_ptrmap(_ptr(slabs), slabs.size, sx, sz, shift, _ptr(out)))
     return out
+
+
+def repack(mips, shift):
+    """Copy of a scene with columns aligned to 2^shift slab words."""
+    arr = mip_array(mips)
+    h = C.c_void_p()
+    check(lib.rlevox_repack(arr, len(mips), shift, C.byref(h)))
+    return HostScene(h.value)
 
 
 def validate(mips):

@@ -208,3 +208,40 @@ def test_validate_rejects_malformed_scenes():
     bad = copy_mips(sc.mips)  # mip sizes do not halve
     bad[1]["sy"] = bad[0]["sy"]
     assert rlevox.validate(bad)[0] != 0
+
+
+@pytest.mark.parametrize("shift", [1, 3, 4])
+def test_repack_with_offset_shift_keeps_the_scene(oracle, shift):
+    sc = W.gen("tiny", seed=5)
+    rp = rlevox.repack(sc.mips, shift)
+    assert rlevox.validate(rp.mips)[0] == 0
+    for a, b in zip(sc.mips, rp.mips):
+        assert b["shift"] == shift
+        assert b["slabs"].size >= a["slabs"].size
+        assert np.all((b["ptrmap"][0::2].astype(np.uint64) << shift) < b["slabs"].size)
+        assert np.array_equal(a["ptrmap"][1::2], b["ptrmap"][1::2])
+        assert np.array_equal(rlevox.build_ptrmap(b["slabs"], b["sx"], b["sz"], shift), b["ptrmap"])
+        da, db = oracle.decode(a), oracle.decode(b)
+        assert np.array_equal(da[0], db[0]) and np.array_equal(da[1], db[1])
+    pose = W.random_views(sc.mips[0], 1, seed=4, alt=(4.0, 30.0))[0]
+    ra = oracle.render(sc.mips, pose, 160, 120)
+    rb = oracle.render(rp.mips, pose, 160, 120)
+    for k in ("raybuf", "clip", "colour", "depth"):
+        assert np.array_equal(ra[k], rb[k]), k
+    assert ra["counters"] == rb["counters"]
+    back = rlevox.repack(rp.mips, 0)  # and back to the reference layout
+    for a, b in zip(sc.mips, back.mips):
+        assert np.array_equal(a["slabs"], b["slabs"]) and np.array_equal(a["ptrmap"], b["ptrmap"])
+    with pytest.raises(rlevox.RlevoxError):
+        rlevox.rle4_save("/tmp/never_written.rle4", rp.mips)
+
+
+def test_offset_shift_is_validated():
+    sc = W.gen("tiny", seed=5)
+    rp = rlevox.repack(sc.mips, 2)
+    bad = copy_mips(rp.mips)
+    bad[0]["shift"] = 0  # offsets read without the shift land inside other columns
+    assert rlevox.validate(bad)[0] != 0
+    bad = copy_mips(rp.mips)
+    bad[0]["shift"] = 5
+    assert rlevox.validate(bad)[0] != 0

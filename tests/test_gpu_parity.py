@@ -180,3 +180,16 @@ def test_golden_digests(ctx, c1_scene, tiny_scene, c1_dev, tiny_dev):
         assert G.raybuf_digest(got["raybuf"][:n], got["clip"][:n]) == case["raybuf"], case["name"]
         assert G.digest(got["compose"], got["depth"]) == case["compose"], case["name"]
         assert G.digest(got["colour"]) == case["colour"], case["name"]
+
+
+@pytest.mark.parametrize("shift", [3, 4])
+def test_offset_shift_scene(ctx, oracle, c1_scene, shift):
+    """Columns aligned to 2^shift words (widened offsets) render identically."""
+    rp = rlevox.repack(c1_scene.mips, shift)
+    dev = ctx.upload(rp.mips)
+    try:
+        poses = workloads.flythrough(c1_scene.mips[0], frames=5, seed=3)
+        for p in poses[1:3]:
+            check_view(ctx, oracle, dev, c1_scene.mips, p, 512, 384)
+    finally:
+        dev.free()
