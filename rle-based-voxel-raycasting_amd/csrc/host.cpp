@@ -1,9 +1,12 @@
 // Host-side utilities of librlevox: error reporting, a small thread pool
 // helper, pointer-map rebuild, scene validation, host scenes and the .rle4
 // file format.
+#include <sched.h>
+
 #include <atomic>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -25,10 +28,26 @@ rlevox_status fail(rlevox_status st, const char* fmt, ...) {
 
 void clear_error() { t_error.clear(); }
 
+// Host threads used when a call asks for 0: OMP_NUM_THREADS when set (the
+// process's CPU share on a shared machine), else the CPUs this process may
+// run on.
+int default_threads() {
+  if (const char* e = std::getenv("OMP_NUM_THREADS")) {
+    const int n = std::atoi(e);
+    if (n > 0) return n;
+  }
+  cpu_set_t set;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) {
+    const int n = CPU_COUNT(&set);
+    if (n > 0) return n;
+  }
+  const int n = (int)std::thread::hardware_concurrency();
+  return n > 0 ? n : 1;
+}
+
 void parallel_for(int64_t n, int threads, int64_t chunk, const std::function<void(int64_t)>& body) {
   if (n <= 0) return;
-  if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
-  if (threads <= 0) threads = 1;
+  if (threads <= 0) threads = default_threads();
   if (chunk <= 0) chunk = 1;
   const int64_t nchunks = (n + chunk - 1) / chunk;
   if (threads > nchunks) threads = (int)nchunks;
