@@ -41,6 +41,18 @@ __device__ inline int readlane_i(int v, int l) { return __builtin_amdgcn_readlan
 
 __device__ inline bool lds_bit(const uint32_t* bits, int y) { return (bits[y >> 5] >> (y & 31)) & 1u; }
 
+// LDS hand-off between the lanes of one wavefront.  These kernels run one
+// wavefront per workgroup, and a wavefront's LDS instructions execute in
+// program order, so a write by one lane is seen by a later read of another
+// lane once the compiler keeps both on their side of this point.  Unlike a
+// workgroup barrier, this emits no s_barrier and does not wait for the
+// wavefront's outstanding global loads (the prefetches stay in flight).
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // LDS floats per wavefront used by the DDA chunk.
 constexpr int kDdaLds = 7 * 64;
 // Slab words cached per step (from the 16-byte boundary at or below the
@@ -106,7 +118,7 @@ __device__ inline int draw_span(int y0, int s2, float uzb, float ozb, float u2dz
       const uint32_t hi = off ? (uint32_t)(dm >> (64 - off)) : 0u;
       if (hi) bits[w0 + 2] |= hi;
     }
-    __syncthreads();
+    wave_sync();
     if (base + 64 < s2) {
       uzb = readlane_f(uz, 63) + u2dz;
       ozb = readlane_f(oz, 63) + onedz2;
@@ -137,11 +149,10 @@ struct DdaChunk {
   uint2 ent;
 };
 
-// Advances the walk by one chunk and issues the pointer-map gather of its
-// cells.  Returns false when the line has reached the far distance.
+// Advances the walk by one chunk.  Returns false when the line has reached
+// the far distance.
 __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& sc, int64_t zfar,
-                                      float vpx, float vpz, float fix_xf, float fix_yf, float* dl,
-                                      int lane) {
+                                      float* dl, int lane) {
   // distance mip switch (Cuda_Render.h:343-365), uniform
   while (w.z > (float)w.mapswitch || w.yswitch > 512.0f) {
     w.yswitch = w.yswitch * 0.5f;
@@ -193,7 +204,7 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
     sY0[lane] = seq_term(cY0, lane);
     sX1[lane] = seq_term(cX1, lane);
     sY1[lane] = seq_term(cY1, lane);
-    __syncthreads();
+    wave_sync();
     // merge ranks: the sequences grow by about gd per term
     const int gx = d_f2i(ceilf(__fdividef(ak - w.dd1, w.gd1)));
     const int gz = d_f2i(ceilf(__fdividef(bk - w.dd0, w.gd0)));
@@ -203,7 +214,7 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
     if (rz < L) slot[rz] = 64 + lane;
     const int nx = (int)__popcll(__ballot(rx < L));
     const int nz = L - nx;
-    __syncthreads();
+    wave_sync();
     if (lane < L) {
       const int cc = slot[lane];
       c.dn = cc < 64 ? sA[cc] : sB[cc - 64];
@@ -231,7 +242,7 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
     w.dd1 = nz < 64 ? sB[nz] : sB[63] + w.gd1;
     w.i1x = nz < 64 ? sX1[nz] : sX1[63] + w.g1x;
     w.i1y = nz < 64 ? sY1[nz] : sY1[63] + w.g1y;
-    __syncthreads();
+    wave_sync();
   } else {
     // plain sequential walk (small distances, axis-aligned lines)
     for (int k = 0; k < L; k++) {
@@ -264,7 +275,13 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
     }
   }
   w.z += (float)L * w.dz;
-  // cell of each step (Cuda_Render.h:416-442) and its pointer-map entry
+  return true;
+}
+
+// Cell of each step of the chunk just walked (Cuda_Render.h:416-442) and the
+// load of its pointer-map entry (left in flight).
+__device__ inline void dda_gather(const DdaWalk& w, DdaChunk& c, float vpx, float vpz,
+                                  float fix_xf, float fix_yf, int lane) {
   const int fix_x = (int)((float)(1 - c.ib) * fix_xf);
   const int fix_z = (int)((float)c.ib * fix_yf);
   const int voxel_x = d_f2i(vpx + c.pbx) + fix_x;
@@ -272,8 +289,7 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
   const int vx = (voxel_x >> w.mip) & (w.gridx - 1);
   const int vz = (voxel_z >> w.mip) & (w.gridz - 1);
   c.ent = make_uint2(0u, 0u);
-  if (lane < L) c.ent = w.map[vx + vz * w.gridx];
-  return true;
+  if (lane < c.L) c.ent = w.map[vx + vz * w.gridx];
 }
 
 template <bool COUNT>
@@ -367,7 +383,7 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     clip[0] = ymin;
     clip[1] = ymax;
   }
-  __syncthreads();
+  wave_sync();
   LineCounts cnt;
   if (COUNT) cnt.cleared = (uint64_t)(ymax - ymin + 1);
 
@@ -418,12 +434,14 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
   const int64_t zfar = rm.far_distance;
   const float mountain = vpy;
 
-  // Software pipeline: while chunk c is projected and replayed, the DDA of
-  // chunk c+1 runs and its pointer-map gather is in flight.  The walk does
+  // Software pipeline: while chunk c's run-cache loads are in flight the DDA
+  // of chunk c+1 runs, and while chunk c is replayed the pointer-map gather
+  // of chunk c+1 is in flight.  The walk does
   // not depend on the replay (only the line's end does), so a chunk
   // computed past the end of the line is simply dropped.
   DdaChunk cur;
-  bool have = dda_next_chunk(wk, cur, sc, zfar, vpx, vpz, fix_xf, fix_yf, dl, lane);
+  bool have = dda_next_chunk(wk, cur, sc, zfar, dl, lane);
+  if (have) dda_gather(wk, cur, vpx, vpz, fix_xf, fix_yf, lane);
   while (have) {
     if (ymin >= ymax) break;
     const int L = cur.L, mip = cur.mip, shift = cur.shift;
@@ -497,22 +515,31 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     const int rc_delta = (int)(wofs - rc_base);
     const uint16_t* cached = rc + lane * kRunCache + rc_delta;
     const int ccap = cand ? kRunCache - rc_delta : 0;
-    uint4 rcv[kRunCache / 8];
+    static_assert(kRunCache == 40, "the run cache is fetched as five 16-byte words");
+    uint4 rc0 = make_uint4(0, 0, 0, 0), rc1 = rc0, rc2 = rc0, rc3 = rc0, rc4 = rc0;
     if (cand) {
       const uint4* src = reinterpret_cast<const uint4*>(slab + rc_base);
-#pragma unroll
-      for (int i = 0; i < kRunCache / 8; i++) rcv[i] = src[i];
+      rc0 = src[0];
+      rc1 = src[1];
+      rc2 = src[2];
+      rc3 = src[3];
+      rc4 = src[4];
     }
 
-    // ---- next chunk: DDA and pointer gather
+    // ---- next chunk's DDA (the run cache loads are in flight meanwhile)
     DdaChunk nxt;
-    const bool have_next = dda_next_chunk(wk, nxt, sc, zfar, vpx, vpz, fix_xf, fix_yf, dl, lane);
+    const bool have_next = dda_next_chunk(wk, nxt, sc, zfar, dl, lane);
 
     if (cand) {
       uint4* dst = reinterpret_cast<uint4*>(rc + lane * kRunCache);
-#pragma unroll
-      for (int i = 0; i < kRunCache / 8; i++) dst[i] = rcv[i];
+      dst[0] = rc0;
+      dst[1] = rc1;
+      dst[2] = rc2;
+      dst[3] = rc3;
+      dst[4] = rc4;
     }
+    // ---- next chunk's pointer gather: in flight during this chunk's replay
+    if (have_next) dda_gather(wk, nxt, vpx, vpz, fix_xf, fix_yf, lane);
     // first element's lower end and span parameters (Cuda_Render.h:554-660)
     int s1F = 0;
     bool drawF = false;
@@ -543,7 +570,7 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
         onedz2F /= s2r - s1rF;
       }
     }
-    __syncthreads();  // run cache filled
+    wave_sync();  // run cache filled
     if (cand) {  // F2: projected top of the next element the loop tests
       int yg = ygF;
       for (uint32_t k = (uint32_t)kF + 1; k < slen; k++) {

@@ -120,7 +120,7 @@ def test_queue_limit_cancel_and_timeout(ctx, scenes, c1_scene):
             try:
                 ts.append(srv.submit(scenes[0], p, 3840, 2160))
             except rlevox.RlevoxError as e:
-                assert e.value.name == "QUEUE_FULL"
+                assert e.name == "QUEUE_FULL"
                 full += 1
         assert full >= 1 and len(ts) >= 3
         srv.cancel(ts[-1])
@@ -133,7 +133,7 @@ def test_queue_limit_cancel_and_timeout(ctx, scenes, c1_scene):
         try:
             srv.wait(t, timeout_ms=0)
         except rlevox.RlevoxError as e:
-            assert e.value.name == "NOT_READY"
+            assert e.name == "NOT_READY"
         srv.wait(t)  # the ticket is still valid after a time-out
         with pytest.raises(rlevox.RlevoxError):
             srv.wait(t)  # consumed
