@@ -365,6 +365,15 @@ int orc_raymap(const rlevox_pose* pose, int W, int H, int mip_distance, int far_
   // mapswitch = MIP_DISTANCE * (0.25*(4-abs(viewrot.x))) (Cuda_Render.h:182,335)
   r.mapswitch = (int)((double)mipd * (0.25 * (double)(4.0f - std::fabs(rot.x))));
   if (r.mapswitch < 1) return 1;  // the mip-switch loop could not terminate
+  {  // and its int doublings must not overflow (see the camera-row halvings)
+    int n = 0;
+    float y = r.position[1];
+    while (y > 512.0f && n < 40) {
+      y = y * 0.5f;
+      n++;
+    }
+    if (n >= 40 || ((int64_t)r.mapswitch << n) > (1ll << 30)) return 1;
+  }
 
   // Compose uniforms (main.cpp:575-603), RAYS_CASTED = RAYS_CASTED_RES = 4*rs.
   r.vanish[0] = 1 - r.vp2d[0];

@@ -55,7 +55,7 @@ inline rlevox_status check_raymap(const rlevox_raymap& rm) {
   }
   if (sum != rm.map_line_count) return fail(RLEVOX_ERR_INVALID_ARG, "raymap: line count mismatch");
   if (rm.mapswitch < 1 || rm.mapswitch > (1 << 24) || rm.far_distance < 1 ||
-      rm.far_distance > (1 << 22))
+      rm.far_distance > (1 << 22) || !mapswitch_bounded(rm.mapswitch, rm.position[1]))
     return fail(RLEVOX_ERR_INVALID_ARG, "raymap: mip/far distance out of range");
   return RLEVOX_OK;
 }

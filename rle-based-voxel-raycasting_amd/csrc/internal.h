@@ -22,6 +22,21 @@ void parallel_for(int64_t n, int threads, int64_t chunk, const std::function<voi
 rlevox_status build_ptrmap(const uint16_t* slabs, uint64_t nslabs, int sx, int sz, int shift,
                            uint32_t* out);
 
+// The mip-switch loop (Cuda_Render.h:343-365) doubles the int mapswitch
+// once per halving of the camera row above 512, then while the distance
+// walked (at most the far distance, <= 2^22) exceeds it.  Poses whose
+// doublings would pass 2^30 are rejected, so the int never overflows (an
+// overflowed mapswitch would keep the loop going forever).
+inline bool mapswitch_bounded(int mapswitch, float camera_row) {
+  int n = 0;
+  float y = camera_row;
+  while (y > 512.0f && n < 40) {
+    y = y * 0.5f;
+    n++;
+  }
+  return mapswitch >= 1 && n < 40 && ((int64_t)mapswitch << n) <= (1ll << 30);
+}
+
 struct HostLevel {
   int32_t sx = 0, sy = 0, sz = 0;
   int32_t shift = 0;  // offset_shift of the level

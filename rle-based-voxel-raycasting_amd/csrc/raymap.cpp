@@ -309,6 +309,9 @@ rlevox_status raymap_compute(const rlevox_pose& pose, int W, int H, int mip_dist
   if (rm.mapswitch < 1)
     return fail(RLEVOX_ERR_INVALID_ARG, "raymap: mip distance %d is too small at pitch %g", mipd,
                 (double)pose.rot[0]);
+  if (!mapswitch_bounded(rm.mapswitch, rm.position[1]))
+    return fail(RLEVOX_ERR_INVALID_ARG, "raymap: mip distance %d too large for camera row %g", mipd,
+                (double)rm.position[1]);
 
   const float lines_tex = (float)(4 * W);  // RAYS_CASTED_RES == RAYS_CASTED == 4*rs
   rm.vanish[0] = 1 - rm.vp2d[0];

@@ -107,6 +107,7 @@ def test_map_line_count_and_sizes():
     (W.pose(2e6, -10, 0, 0.3, 0.0), 640, 480, (0, 0), "position too far out"),
     (W.pose(0, -10, 0, 1.5, 0.0), 640, 480, (1, 0), "mip distance gives mapswitch < 1"),
     (W.pose(0, -10, 0, 0.3, 0.0), 640, 480, (0, 1 << 23), "far distance too large"),
+    (W.pose(0, 9e5, 0, 0.3, 0.0), 640, 480, (1 << 20, 0), "mip-switch doublings would overflow"),
 ])
 def test_rejected_poses(pose, W, H, params, why):
     with pytest.raises(rlevox.RlevoxError) as e:
