@@ -3,14 +3,12 @@
 //
 //  1. DDA chunk: up to 64 DDA steps per pass, lane k taking step k.  The
 //     crossing distances and positions are repeated fp32 additions
-//     (Cuda_Render.h:398-414); within one binade such a sum advances by a
-//     fixed number of ulps, so each lane computes its term from a few
-//     chunk-wide integers (SeqChunk), and the x/z choice order is the merge
-//     order of the two crossing-distance sequences (ties go to x, because
-//     the walk takes z only when it is strictly closer).  Chunks where a
-//     sequence could cross two binades fall back to the plain loop.
+//     (Cuda_Render.h:398-414) of six independent sequences; their terms
+//     0..63 are computed side by side with the same additions the walk
+//     makes, and the x/z order is the merge order of the two distance
+//     sequences (ties go to x: the walk takes z only when strictly closer).
 //  2. Gather: each lane loads its cell's pointer-map entry, one round trip
-//     for the whole chunk.
+//     for the whole chunk, issued one chunk ahead (software pipeline).
 //  3. Stateless projection per lane: clip-top inputs and F, the projected
 //     top of the first drawn element in front of the eye.
 //  4. Run cache: each step that may act copies the head of its column
@@ -30,7 +28,6 @@
 #pragma once
 
 #include "device.h"
-#include "ffadd.h"
 
 namespace rlevox {
 
@@ -134,6 +131,7 @@ struct DdaWalk {
   float px, py, dnow;                  // last step taken
   int idx;
   int mip, gridx, gridz, mapswitch, shift;
+  int dzlog;  // dz = 2^dzlog
   float dz, z, yswitch;
   const uint2* map;
   const uint16_t* slab;
@@ -151,7 +149,7 @@ struct DdaChunk {
 
 // Advances the walk by one chunk.  Returns false when the line has reached
 // the far distance.
-__device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& sc, int64_t zfar,
+__device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& sc, int zfar,
                                       float* dl, int lane) {
   // distance mip switch (Cuda_Render.h:343-365), uniform
   while (w.z > (float)w.mapswitch || w.yswitch > 512.0f) {
@@ -172,38 +170,63 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
     w.gd1 *= 2;
     w.mapswitch *= 2;
     w.dz *= 2;
+    w.dzlog++;
   }
-  // Iterations left before the next mip switch and before z passes z_far
-  // (z, dz and mapswitch are exact integers).
-  const int64_t zi = (int64_t)w.z, dzi = (int64_t)w.dz;
-  const int64_t n_mip = ((int64_t)w.mapswitch - zi) / dzi + 1;
-  const int64_t n_far = (zfar - zi) / dzi;
+  // Iterations left before the next mip switch and before z passes z_far.
+  // z, dz = 2^dzlog and mapswitch are exact integers below 2^31, and
+  // z <= mapswitch here; a negative distance to z_far ends the line either
+  // way, so the shifts decide exactly as the divisions would.
+  const int zi = (int)w.z;
+  const int n_mip = ((w.mapswitch - zi) >> w.dzlog) + 1;
+  const int n_far = (zfar - zi) >> w.dzlog;
   if (n_far <= 0) return false;
-  const int L = (int)min((int64_t)64, min(n_mip, n_far));
+  const int L = min(64, min(n_mip, n_far));
   c.L = L;
   c.mip = w.mip;
   c.shift = w.shift;
   c.slab = w.slab;
   c.db = c.dn = c.pbx = c.pby = 0;
   c.ib = 0;
-  const SeqChunk cA = seq_chunk(w.dd0, w.gd0), cB = seq_chunk(w.dd1, w.gd1);
-  const SeqChunk cX0 = seq_chunk(w.i0x, w.g0x), cY0 = seq_chunk(w.i0y, w.g0y);
-  const SeqChunk cX1 = seq_chunk(w.i1x, w.g1x), cY1 = seq_chunk(w.i1y, w.g1y);
-  if (cA.ok && cB.ok && cX0.ok && cY0.ok && cX1.ok && cY1.ok) {
-    float* sA = dl;
-    float* sB = dl + 64;
-    float* sX0 = dl + 128;
-    float* sY0 = dl + 192;
-    float* sX1 = dl + 256;
-    float* sY1 = dl + 320;
-    int* slot = reinterpret_cast<int*>(dl + 384);
-    const float ak = seq_term(cA, lane), bk = seq_term(cB, lane);
+  // The six crossing sequences (distances, x and z positions of the x and z
+  // crossings) are independent recurrences s <- s + g: their terms 0..63 are
+  // computed side by side, each by the same additions the sequential walk
+  // makes (the k-th x-crossing term has seen k additions whatever the
+  // interleaving), and lane k keeps term k.  The walk's order of x and z
+  // steps is then the merge of the two distance sequences (ties go to x:
+  // the walk takes z only when strictly closer).
+  float* sA = dl;
+  float* sB = dl + 64;
+  float* sX0 = dl + 128;
+  float* sY0 = dl + 192;
+  float* sX1 = dl + 256;
+  float* sY1 = dl + 320;
+  int* slot = reinterpret_cast<int*>(dl + 384);
+  {
+    float ta = w.dd0, tb = w.dd1, tx0 = w.i0x, ty0 = w.i0y, tx1 = w.i1x, ty1 = w.i1y;
+    float ak = ta, bk = tb, x0k = tx0, y0k = ty0, x1k = tx1, y1k = ty1;
+#pragma unroll
+    for (int k = 1; k < 64; k++) {
+      ta = ta + w.gd0;
+      tb = tb + w.gd1;
+      tx0 = tx0 + w.g0x;
+      ty0 = ty0 + w.g0y;
+      tx1 = tx1 + w.g1x;
+      ty1 = ty1 + w.g1y;
+      if (lane == k) {
+        ak = ta;
+        bk = tb;
+        x0k = tx0;
+        y0k = ty0;
+        x1k = tx1;
+        y1k = ty1;
+      }
+    }
     sA[lane] = ak;
     sB[lane] = bk;
-    sX0[lane] = seq_term(cX0, lane);
-    sY0[lane] = seq_term(cY0, lane);
-    sX1[lane] = seq_term(cX1, lane);
-    sY1[lane] = seq_term(cY1, lane);
+    sX0[lane] = x0k;
+    sY0[lane] = y0k;
+    sX1[lane] = x1k;
+    sY1[lane] = y1k;
     wave_sync();
     // merge ranks: the sequences grow by about gd per term
     const int gx = d_f2i(ceilf(__fdividef(ak - w.dd1, w.gd1)));
@@ -243,36 +266,6 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
     w.i1x = nz < 64 ? sX1[nz] : sX1[63] + w.g1x;
     w.i1y = nz < 64 ? sY1[nz] : sY1[63] + w.g1y;
     wave_sync();
-  } else {
-    // plain sequential walk (small distances, axis-aligned lines)
-    for (int k = 0; k < L; k++) {
-      const float db = w.dnow, pbx = w.px, pby = w.py;
-      const int ib = w.idx;
-      if (w.dd1 < w.dd0) {
-        w.dnow = w.dd1;
-        w.idx = 1;
-        w.dd1 += w.gd1;
-        w.px = w.i1x;
-        w.py = w.i1y;
-        w.i1x += w.g1x;
-        w.i1y += w.g1y;
-      } else {
-        w.dnow = w.dd0;
-        w.idx = 0;
-        w.px = w.i0x;
-        w.py = w.i0y;
-        w.dd0 += w.gd0;
-        w.i0x += w.g0x;
-        w.i0y += w.g0y;
-      }
-      if (k == lane) {
-        c.db = db;
-        c.dn = w.dnow;
-        c.pbx = pbx;
-        c.pby = pby;
-        c.ib = ib;
-      }
-    }
   }
   w.z += (float)L * w.dz;
   return true;
@@ -427,11 +420,12 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
   wk.shift = sc.mip[0].shift;
   wk.mapswitch = rm.mapswitch;
   wk.dz = 1;
+  wk.dzlog = 0;
   wk.z = 0;
   wk.yswitch = vpy;
   const float z_add = sin_x;
   const float y_add = (vertical ? cos_x : 0.0f) * hmul;
-  const int64_t zfar = rm.far_distance;
+  const int zfar = rm.far_distance;
   const float mountain = vpy;
 
   // Software pipeline: while chunk c's run-cache loads are in flight the DDA
