@@ -170,7 +170,8 @@ typedef struct {
                               consecutive DDA steps); 16, 32 or 64 = one lane
                               per line */
   int32_t waves_per_block; /* 1..4 */
-  int32_t prefetch;        /* pointer-map entries fetched ahead per line (0..4) */
+  int32_t persistent_waves; /* wavefronts per CU of the persistent batch kernel
+                               (0 = default 16, else 4..32) */
   int32_t order;           /* 0 = view order, 1 = longest views first */
 } rlevox_tuning;
 

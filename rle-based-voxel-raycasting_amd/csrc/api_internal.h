@@ -26,7 +26,7 @@ rlevox_status hip_fail(hipError_t e, const char* what);
 inline bool tuning_valid(const rlevox_tuning& t) {
   return (t.lines_per_wave == 1 || t.lines_per_wave == 16 || t.lines_per_wave == 32 ||
           t.lines_per_wave == 64) &&
-         t.waves_per_block >= 1 && t.waves_per_block <= 4 && t.prefetch >= 0 && t.prefetch <= 4 &&
+         t.waves_per_block >= 1 && t.waves_per_block <= 4 && (t.persistent_waves == 0 || (t.persistent_waves >= 4 && t.persistent_waves <= 32)) &&
          (t.order == 0 || t.order == 1);
 }
 
@@ -76,6 +76,8 @@ struct BatchSlot {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int nviews = 0;
+  int waves = kWavesPerCU;        // persistent wavefronts per CU of this batch
+  int graph_waves = 0;            // the value the captured graph was made with
   uint64_t rays = 0;
   rlevox_status init(rlevox_ctx* c);
   rlevox_status prepare(const rlevox_view* views, int n, const rlevox_render_params* params,

@@ -38,8 +38,8 @@ __device__ inline int find_view(const ViewDesc* __restrict__ v, int n, int64_t g
 __global__ __launch_bounds__(64) void k_raycast_batch(BatchHeader* __restrict__ h,
                                                       const ViewDesc* __restrict__ views) {
   __shared__ uint32_t bits[(RLEVOX_MAX_RS + 31) / 32];
-  __shared__ float dda[kDdaLds];
   __shared__ uint4 rcache[64 * kRunCache / 8];
+  float* dda = reinterpret_cast<float*>(rcache);  // see kDdaLds
   const int lane = threadIdx.x;
   const int64_t total = h->total_lines;
   const int nv = h->nviews;
@@ -152,6 +152,7 @@ rlevox_status BatchSlot::prepare(const rlevox_view* views, int n, const rlevox_r
   }
   std::vector<int> order(n);
   std::iota(order.begin(), order.end(), 0);
+  waves = tuning.persistent_waves ? tuning.persistent_waves : kWavesPerCU;
   if (tuning.order == 1)
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
   // scratch layout: per view ray buffer, line clip ranges, composed colour
@@ -226,13 +227,20 @@ rlevox_status BatchSlot::run(bool use_graph, hipStream_t s) {
   hipError_t e = hipMemcpyAsync(descs.ptr, staging, copy, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return hip_fail(e, "batch descriptors");
   const int cus = ctx->num_cus > 0 ? ctx->num_cus : 256;
-  const int ray_grid = cus * kWavesPerCU;
+  const int ray_grid = cus * waves;
   const int post_grid = cus * 8;
   if (!use_graph) {
     e = launch_batch(d_hdr, d_views, ray_grid, post_grid, s);
     return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "batch launch");
   }
+  if (exec && graph_waves != waves) {  // launch shape changed: capture again
+    hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
   if (!exec) {
+    graph_waves = waves;
     hipStream_t cap = nullptr;
     e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);

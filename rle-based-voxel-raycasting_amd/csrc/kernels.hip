@@ -30,8 +30,8 @@ __global__ __launch_bounds__(64) void k_raycast_wave(SceneDev sc, rlevox_raymap 
                                                      uint32_t* __restrict__ raybuf,
                                                      int32_t* __restrict__ clip,
                                                      unsigned long long* counts) {
-  __shared__ float dda[kDdaLds];
   __shared__ uint4 rcache[64 * kRunCache / 8];
+  float* dda = reinterpret_cast<float*>(rcache);  // see kDdaLds
   extern __shared__ uint32_t lds[];
   const int x = blockIdx.x;
   if (x >= rm.map_line_count) return;

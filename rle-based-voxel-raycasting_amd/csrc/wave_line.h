@@ -50,11 +50,14 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// LDS floats per wavefront used by the DDA chunk.
-constexpr int kDdaLds = 7 * 64;
 // Slab words cached per step (from the 16-byte boundary at or below the
 // column header): covers header, elements and colours of typical columns.
 constexpr int kRunCache = 40;
+// LDS floats per wavefront used by the DDA chunk.  The kernels place them in
+// the run cache's memory: the DDA of chunk c+1 runs after chunk c's replay
+// has finished with the cache and before chunk c+1's cache is stored.
+constexpr int kDdaLds = 7 * 64;
+static_assert(kDdaLds * 4 <= 64 * kRunCache * 2, "DDA scratch must fit in the run cache");
 
 // First y in [from, to) whose bit is clear, or `to` (from <= to).
 __device__ inline int first_clear(const uint32_t* bits, int from, int to, int lane) {
@@ -78,6 +81,21 @@ __device__ inline int rank_in(const float* arr, float v, int guess) {
 }
 
 // ------------------------------------------------------------ span pixels
+
+// Sets the bitmap bits of rows base..base+63 given by mask m.  Lane 0 issues
+// LDS atomic ORs whose results nobody waits for.  Only rows below the span
+// end (< rs) are ever in m, so the words stay inside the bitmap.
+__device__ inline void bits_or(uint32_t* bits, int base, uint64_t m, int lane) {
+  if (m && lane == 0) {
+    const int w0 = base >> 5, off = base & 31;
+    const uint32_t lo = (uint32_t)(m << off);
+    if (lo) atomicOr(&bits[w0], lo);
+    const uint32_t mid = off ? (uint32_t)(m >> (32 - off)) : (uint32_t)(m >> 32);
+    if (mid) atomicOr(&bits[w0 + 1], mid);
+    const uint32_t hi = off ? (uint32_t)(m >> (64 - off)) : 0u;
+    if (hi) atomicOr(&bits[w0 + 2], hi);
+  }
+}
 
 // Draws pixels [y0, s2) of one span (uniform arguments; Cuda_Render.h:
 // 678-733).  Pixel y0+k has had k per-pixel increments.
@@ -105,16 +123,7 @@ __device__ inline int draw_span(int y0, int s2, float uzb, float ozb, float u2dz
     }
     const uint64_t dm = __ballot(draw);
     drawn += (int)__popcll(dm);
-    if (dm && lane == 0) {
-      // pixel base+i is bit i of dm; spread over at most three words
-      const int w0 = base >> 5, off = base & 31;
-      const uint64_t lo = dm << off;
-      bits[w0] |= (uint32_t)lo;
-      const uint32_t mid = off ? (uint32_t)(dm >> (32 - off)) : (uint32_t)(dm >> 32);
-      if (mid) bits[w0 + 1] |= mid;
-      const uint32_t hi = off ? (uint32_t)(dm >> (64 - off)) : 0u;
-      if (hi) bits[w0 + 2] |= hi;
-    }
+    bits_or(bits, base, dm, lane);  // pixel base+i is bit i of dm
     wave_sync();
     if (base + 64 < s2) {
       uzb = readlane_f(uz, 63) + u2dz;
@@ -122,6 +131,57 @@ __device__ inline int draw_span(int y0, int s2, float uzb, float ozb, float u2dz
     }
   }
   return drawn;
+}
+
+// The clear rows of a span [s1, e2): for spans of at most 64 rows one LDS
+// read and one ballot give both the first clear row y0 (which fixes the
+// span's start value, Cuda_Render.h:678-680) and the rows to draw.
+struct SpanWindow {
+  int y0;      // first clear row, or e2
+  uint64_t m;  // fused: clear rows, bit j = row s1 + j
+  bool fused;
+};
+
+__device__ inline SpanWindow span_window(const uint32_t* bits, int s1, int e2, int lane) {
+  SpanWindow w;
+  if (e2 - s1 <= 64) {
+    const bool clear = lane < e2 - s1 && !lds_bit(bits, s1 + lane);
+    w.m = __ballot(clear);
+    w.y0 = w.m ? s1 + (int)__builtin_ctzll(w.m) : e2;
+    w.fused = true;
+  } else {
+    w.y0 = first_clear(bits, s1, e2, lane);
+    w.m = 0;
+    w.fused = false;
+  }
+  return w;
+}
+
+// Draws the clear rows of [w.y0, e2) (Cuda_Render.h:687-733): pixel y0+k has
+// had k per-pixel increments.
+__device__ inline int span_draw(const SpanWindow& w, int s1, int e2, float uzb, float ozb,
+                                float u2dz, float onedz2, float tlo, float thi,
+                                const uint16_t* cached, int ccap, const uint16_t* colours,
+                                int cofs, uint32_t* __restrict__ row, uint32_t* bits, int lane) {
+  if (!w.fused)
+    return draw_span(w.y0, e2, uzb, ozb, u2dz, onedz2, tlo, thi, cached, ccap, colours, cofs, row,
+                     bits, lane);
+  if ((w.m >> lane) & 1) {
+    float uz = uzb, oz = ozb;
+    for (int i = w.y0 - s1; i < lane; i++) {
+      uz += u2dz;
+      oz += onedz2;
+    }
+    const float fu = fminf(fmaxf(uz / oz, tlo), thi);
+    const uint32_t u = (uint32_t)d_f2i(fu);
+    const uint32_t depth = (uint32_t)d_f2i(1.0f / oz) & 0xfffeu;
+    const int cw = cofs + (int)u;
+    const uint32_t col = cw < ccap ? cached[cw] : colours[u];
+    row[s1 + lane] = col + (depth << 16);
+  }
+  bits_or(bits, s1, w.m, lane);
+  wave_sync();
+  return (int)__popcll(w.m);
 }
 
 // Uniform DDA walk state of one line (Cuda_Render.h:270-367).
@@ -188,12 +248,13 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
   c.db = c.dn = c.pbx = c.pby = 0;
   c.ib = 0;
   // The six crossing sequences (distances, x and z positions of the x and z
-  // crossings) are independent recurrences s <- s + g: their terms 0..63 are
-  // computed side by side, each by the same additions the sequential walk
-  // makes (the k-th x-crossing term has seen k additions whatever the
-  // interleaving), and lane k keeps term k.  The walk's order of x and z
-  // steps is then the merge of the two distance sequences (ties go to x:
-  // the walk takes z only when strictly closer).
+  // crossings) are independent recurrences s <- s + g.  Lane j < 6 walks
+  // sequence j for 63 additions, storing every term to LDS (one add and one
+  // store per term for all six sequences at once); each term is made by the
+  // same additions the sequential walk makes (the k-th x-crossing term has
+  // seen k additions whatever the interleaving).  Lane k then reads term k.
+  // The walk's order of x and z steps is the merge of the two distance
+  // sequences (ties go to x: the walk takes z only when strictly closer).
   float* sA = dl;
   float* sB = dl + 64;
   float* sX0 = dl + 128;
@@ -202,32 +263,34 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
   float* sY1 = dl + 320;
   int* slot = reinterpret_cast<int*>(dl + 384);
   {
-    float ta = w.dd0, tb = w.dd1, tx0 = w.i0x, ty0 = w.i0y, tx1 = w.i1x, ty1 = w.i1y;
-    float ak = ta, bk = tb, x0k = tx0, y0k = ty0, x1k = tx1, y1k = ty1;
+    if (lane < 6) {
+      float t = w.dd0, g = w.gd0;
+      if (lane == 1) {
+        t = w.dd1;
+        g = w.gd1;
+      } else if (lane == 2) {
+        t = w.i0x;
+        g = w.g0x;
+      } else if (lane == 3) {
+        t = w.i0y;
+        g = w.g0y;
+      } else if (lane == 4) {
+        t = w.i1x;
+        g = w.g1x;
+      } else if (lane == 5) {
+        t = w.i1y;
+        g = w.g1y;
+      }
+      float* dst = dl + lane * 64;
+      dst[0] = t;
 #pragma unroll
-    for (int k = 1; k < 64; k++) {
-      ta = ta + w.gd0;
-      tb = tb + w.gd1;
-      tx0 = tx0 + w.g0x;
-      ty0 = ty0 + w.g0y;
-      tx1 = tx1 + w.g1x;
-      ty1 = ty1 + w.g1y;
-      if (lane == k) {
-        ak = ta;
-        bk = tb;
-        x0k = tx0;
-        y0k = ty0;
-        x1k = tx1;
-        y1k = ty1;
+      for (int k = 1; k < 64; k++) {
+        t = t + g;
+        dst[k] = t;
       }
     }
-    sA[lane] = ak;
-    sB[lane] = bk;
-    sX0[lane] = x0k;
-    sY0[lane] = y0k;
-    sX1[lane] = x1k;
-    sY1[lane] = y1k;
     wave_sync();
+    const float ak = sA[lane], bk = sB[lane];
     // merge ranks: the sequences grow by about gd per term
     const int gx = d_f2i(ceilf(__fdividef(ak - w.dd1, w.gd1)));
     const int gz = d_f2i(ceilf(__fdividef(bk - w.dd0, w.gd0)));
@@ -653,17 +716,17 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
             ymin = e2;
             if (ymax > ymin) ymin = first_clear(bits, ymin, ymax, lane);
           }
-          const int y0 = first_clear(bits, s1, e2, lane);
-          if (y0 < e2) {
+          const SpanWindow sw = span_window(bits, s1, e2, lane);
+          if (sw.y0 < e2) {
             const float s1r = readlane_f(s1rF, t);
             const float u1z = readlane_f(u1zF, t), u2dz = readlane_f(u2dzF, t);
             const float onez1 = readlane_f(onez1F, t), onedz2 = readlane_f(onedz2F, t);
-            const float mult = (float)(y0 + 1) - s1r;
+            const float mult = (float)(sw.y0 + 1) - s1r;
             const float uzb = u1z + u2dz * mult;
             const float ozb = onez1 + onedz2 * mult;
             const float thi = readlane_f(texF, t) - 1.0f;
             const float tlo = (float)readlane_i(textureF, t);
-            draw_span(y0, e2, uzb, ozb, u2dz, onedz2, tlo, thi, tcache, tcap, tcol,
+            span_draw(sw, s1, e2, uzb, ozb, u2dz, onedz2, tlo, thi, tcache, tcap, tcol,
                       2 + (int)tslen, row, bits, lane);
           }
         }
@@ -728,8 +791,9 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
           ymin = s2;
           if (ymax > ymin) ymin = first_clear(bits, ymin, ymax, lane);
         }
-        const int y0 = first_clear(bits, s1, s2, lane);
-        if (y0 >= s2) continue;
+        const SpanWindow sw = span_window(bits, s1, s2, lane);
+        if (sw.y0 >= s2) continue;
+        const int y0 = sw.y0;
         const float pz1r = tpz + z_add * ytop;
         const float py1r = tpy + y_add * ytop;
         const float pz2r = tpz + z_add * ybot;
@@ -746,8 +810,8 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
         const float uzb = u1z + u2dz * mult;
         const float ozb = onez1 + onedz2 * mult;
         if (COUNT) cnt.spans++;
-        const int d = draw_span(y0, s2, uzb, ozb, u2dz, onedz2, (float)texture, tex - 1.0f, tcache,
-                                tcap, tcol, 2 + (int)tslen, row, bits, lane);
+        const int d = span_draw(sw, s1, s2, uzb, ozb, u2dz, onedz2, (float)texture, tex - 1.0f,
+                                tcache, tcap, tcol, 2 + (int)tslen, row, bits, lane);
         if (COUNT) cnt.col += (uint64_t)d;
       }
       cur_step = t + 1;

@@ -98,10 +98,10 @@ class View(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("lines_per_wave", C.c_int32), ("waves_per_block", C.c_int32),
-                ("prefetch", C.c_int32), ("order", C.c_int32)]
+                ("persistent_waves", C.c_int32), ("order", C.c_int32)]
 
     def as_tuple(self):
-        return (self.lines_per_wave, self.waves_per_block, self.prefetch, self.order)
+        return (self.lines_per_wave, self.waves_per_block, self.persistent_waves, self.order)
 
 
 class ServerConfig(C.Structure):
@@ -416,8 +416,8 @@ class Context:
         check(lib.rlevox_event_elapsed_ms(self.handle, a, b, C.byref(ms)))
         return float(ms.value)
 
-    def set_tuning(self, lines_per_wave=1, waves_per_block=1, prefetch=0, order=1):
-        t = Tuning(lines_per_wave, waves_per_block, prefetch, order)
+    def set_tuning(self, lines_per_wave=1, waves_per_block=1, persistent_waves=0, order=1):
+        t = Tuning(lines_per_wave, waves_per_block, persistent_waves, order)
         check(lib.rlevox_set_tuning(self.handle, C.byref(t)))
 
     def get_tuning(self):
