@@ -44,6 +44,19 @@ def raycast_bytes(c):
             + 4 * (c["cleared"] + c["col_loads"]))
 
 
+def measured_traffic():
+    """HBM bytes per raycast launch of this workload, measured by the
+    rocprofv3 FETCH_SIZE/WRITE_SIZE passes summarised in profiles/."""
+    path = os.path.join(ROOT, "profiles", "k_raycast_wave_c2_1080p_pmc.txt")
+    try:
+        for line in open(path):
+            if line.startswith("JSON "):
+                return json.loads(line[5:])["traffic_bytes"]
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
+
+
 def post_bytes(W, H):
     # compose: read one texel, write colour16 + depth32; gap-fill: read
     # colour16 + depth32, write colour16 (taps hit cache).
@@ -167,8 +180,10 @@ def main():
                    "post_ms_mean": round(float(np.mean(post_ms)), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": None, "kernel": "k_raycast_wave",
-                     "bytes_per_launch": round(mean_bytes)},
+                     "traffic": measured_traffic(), "kernel": "k_raycast_wave",
+                     "bytes_per_launch": round(mean_bytes),
+                     "traffic_source": "profiles/k_raycast_wave_c2_1080p_pmc.txt (rocprofv3 "
+                                       "FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)"},
     }
 
     if not args.no_batch:
