@@ -36,6 +36,17 @@ __device__ inline float readlane_f(float v, int l) {
 }
 __device__ inline int readlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
+// Marks a value that is the same in every lane as uniform, so the compiler
+// keeps it in a scalar register and branches on it with scalar branches
+// instead of exec masks (only for values that really are uniform).
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline float uni_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ inline uint64_t uni64(uint64_t v) {
+  return ((uint64_t)(uint32_t)uni((int)(v >> 32)) << 32) | (uint32_t)uni((int)(uint32_t)v);
+}
+
 __device__ inline bool lds_bit(const uint32_t* bits, int y) { return (bits[y >> 5] >> (y & 31)) & 1u; }
 
 // LDS hand-off between the lanes of one wavefront.  These kernels run one
@@ -63,8 +74,8 @@ static_assert(kDdaLds * 4 <= 64 * kRunCache * 2, "DDA scratch must fit in the ru
 __device__ inline int first_clear(const uint32_t* bits, int from, int to, int lane) {
   for (int base = from; base < to; base += 64) {
     const int y = base + lane;
-    const uint64_t m = __ballot(y < to && !lds_bit(bits, y));
-    if (m) return base + __builtin_ctzll(m);
+    const uint64_t m = uni64(__ballot(y < to && !lds_bit(bits, y)));
+    if (m) return uni(base + __builtin_ctzll(m));
   }
   return to;
 }
@@ -146,8 +157,8 @@ __device__ inline SpanWindow span_window(const uint32_t* bits, int s1, int e2, i
   SpanWindow w;
   if (e2 - s1 <= 64) {
     const bool clear = lane < e2 - s1 && !lds_bit(bits, s1 + lane);
-    w.m = __ballot(clear);
-    w.y0 = w.m ? s1 + (int)__builtin_ctzll(w.m) : e2;
+    w.m = uni64(__ballot(clear));
+    w.y0 = uni(w.m ? s1 + (int)__builtin_ctzll(w.m) : e2);
     w.fused = true;
   } else {
     w.y0 = first_clear(bits, s1, e2, lane);
@@ -317,17 +328,17 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
         c.ib = p < 64 ? 0 : 1;
       }
     }
-    const int cl = slot[L - 1];
-    w.dnow = cl < 64 ? sA[cl] : sB[cl - 64];
-    w.px = cl < 64 ? sX0[cl] : sX1[cl - 64];
-    w.py = cl < 64 ? sY0[cl] : sY1[cl - 64];
+    const int cl = uni(slot[L - 1]);
+    w.dnow = uni_f(cl < 64 ? sA[cl] : sB[cl - 64]);
+    w.px = uni_f(cl < 64 ? sX0[cl] : sX1[cl - 64]);
+    w.py = uni_f(cl < 64 ? sY0[cl] : sY1[cl - 64]);
     w.idx = cl < 64 ? 0 : 1;
-    w.dd0 = nx < 64 ? sA[nx] : sA[63] + w.gd0;
-    w.i0x = nx < 64 ? sX0[nx] : sX0[63] + w.g0x;
-    w.i0y = nx < 64 ? sY0[nx] : sY0[63] + w.g0y;
-    w.dd1 = nz < 64 ? sB[nz] : sB[63] + w.gd1;
-    w.i1x = nz < 64 ? sX1[nz] : sX1[63] + w.g1x;
-    w.i1y = nz < 64 ? sY1[nz] : sY1[63] + w.g1y;
+    w.dd0 = uni_f(nx < 64 ? sA[nx] : sA[63] + w.gd0);
+    w.i0x = uni_f(nx < 64 ? sX0[nx] : sX0[63] + w.g0x);
+    w.i0y = uni_f(nx < 64 ? sY0[nx] : sY0[63] + w.g0y);
+    w.dd1 = uni_f(nz < 64 ? sB[nz] : sB[63] + w.gd1);
+    w.i1x = uni_f(nz < 64 ? sX1[nz] : sX1[63] + w.g1x);
+    w.i1y = uni_f(nz < 64 ? sY1[nz] : sY1[63] + w.g1y);
     wave_sync();
   }
   w.z += (float)L * w.dz;
@@ -656,14 +667,16 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     int cur_step = 0;
     bool done = false;
     for (;;) {
+      ymin = uni(ymin);
+      ymax = uni(ymax);
       if (ymin >= ymax) {
         done = true;
         break;
       }
       const bool culled = top <= pz * (float)ymin;
       const bool act = valid && lane >= cur_step && !culled && F > ymin;
-      const uint64_t am = __ballot(act);
-      const int t = am ? (int)__builtin_ctzll(am) : L;
+      const uint64_t am = uni64(__ballot(act));
+      const int t = uni(am ? (int)__builtin_ctzll(am) : L);
       if (COUNT) {
         // no-op steps [cur_step, t): pointer loads and elements walked to the break
         const bool noop_loaded = valid && lane >= cur_step && lane < t && !culled;
