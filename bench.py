@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--c3", action="store_true",
+                    help="also time config 3: 16384^2 x 1024 map, 3840x2160 grazing views")
     ap.add_argument("--serve-requests", type=int, default=1500)
     ap.add_argument("--serve-rates", type=int, nargs="*", default=[500, 1000, 2000])
     ap.add_argument("--serve-batch", type=int, default=48)
@@ -207,6 +209,13 @@ def main():
         for sc in scenes[1:]:
             sc.free()
 
+    if args.c3:
+        try:
+            result["c3"] = bench_c3(ctx, rlevox, workloads)
+        except Exception as e:
+            log("c3 leg failed: %s" % e)
+            result["c3"] = {"error": str(e)}
+
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(hs, poses, W, H, args.cpu_seconds)
 
@@ -261,6 +270,55 @@ def bench_batched(ctx, rlevox, workloads, scenes_h, scenes):
     log("batched: %s" % out)
     for b in bufs:
         b.free()
+    return out
+
+
+def bench_c3(ctx, rlevox, workloads, frames=8, reps=3):
+    """Config 3: plain kernel on a 16384x16384x1024 map (mips 0-7, level 0
+    beyond 2^32 slab words) at 3840x2160, grazing poses |pitch| 0.6-5 deg."""
+    W, H = 3840, 2160
+    t0 = time.time()
+    hs = workloads.gen("c3", seed=1)
+    log("generated c3 scene in %.1fs (%.2f GB host, level-0 offset shift %d)"
+        % (time.time() - t0, hs.nbytes / 1e9, hs.mips[0]["shift"]))
+    scene = ctx.upload(hs.mips)
+    poses = workloads.grazing(hs.mips[0], n=frames, seed=11)
+    rms = [rlevox.raymap(p, W, H) for p in poses]
+    lines = max(r.map_line_count for r in rms)
+    d_rb, d_clip = ctx.alloc(lines * W * 4), ctx.alloc(lines * 8)
+    d_c0, d_col, d_dep = ctx.alloc(W * H * 2), ctx.alloc(W * H * 2), ctx.alloc(W * H * 4)
+    counts = [ctx.raycast_counters(scene, rm, d_rb, d_clip) for rm in rms]
+    stream = ctx.stream()
+    ev = [(ctx.event(), ctx.event()) for _ in range(frames * reps)]
+    for rm in rms:  # warm-up
+        ctx.render_raybuf(scene, rm, d_rb, d_clip, stream)
+    ctx.sync(stream)
+    t0 = time.perf_counter()
+    for r in range(reps):
+        for i, p in enumerate(poses):
+            rm = rlevox.raymap(p, W, H)
+            ctx.record(ev[r * frames + i][0], stream)
+            ctx.render_raybuf(scene, rm, d_rb, d_clip, stream)
+            ctx.record(ev[r * frames + i][1], stream)
+            ctx.compose(rm, d_rb, d_clip, d_c0, d_dep, stream)
+            ctx.gapfill(W, H, d_c0, d_dep, d_col, stream)
+    ctx.sync(stream)
+    wall = time.perf_counter() - t0
+    ray_ms = float(np.mean([ctx.elapsed_ms(a, b) for a, b in ev]))
+    mean_bytes = float(np.mean([raycast_bytes(c) for c in counts]))
+    out = {"workload": "config 3: 3840x2160 grazing views (8 seeded poses, |pitch| 0.6-5 deg) over "
+                       "a resident 16384x16384x1024 RLE terrain, mips 0-7",
+           "scene_gb": round(scene.device_bytes / 1e9, 2), "offset_shift": hs.mips[0]["shift"],
+           "mrays_s": round(frames * reps * W * H / wall / 1e6, 2),
+           "ms_per_frame": round(wall * 1e3 / (frames * reps), 3),
+           "raycast_ms_mean": round(ray_ms, 3),
+           "raycast_bytes_per_launch": round(mean_bytes),
+           "raycast_gbs": round(mean_bytes / (ray_ms * 1e-3) / 1e9, 2)}
+    log("c3: %s" % out)
+    for b in (d_rb, d_clip, d_c0, d_col, d_dep):
+        b.free()
+    scene.free()
+    hs.close()
     return out
 
 
