@@ -39,6 +39,7 @@ __global__ __launch_bounds__(64) void k_raycast_batch(BatchHeader* __restrict__ 
                                                       const ViewDesc* __restrict__ views) {
   __shared__ uint32_t bits[(RLEVOX_MAX_RS + 31) / 32];
   __shared__ uint4 rcache[64 * kRunCache / 8];
+  __shared__ int sweep[64];
   float* dda = reinterpret_cast<float*>(rcache);  // see kDdaLds
   const int lane = threadIdx.x;
   const int64_t total = h->total_lines;
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(64) void k_raycast_batch(BatchHeader* __restrict__ 
     const int x = (int)((int64_t)g - vd.line_begin);
     render_line_wave<false>(x, *vd.scene, vd.rm, vd.raybuf + (size_t)x * (size_t)vd.rm.rs,
                             vd.clip + 2 * x, bits, dda, reinterpret_cast<uint16_t*>(rcache),
-                            nullptr);
+                            sweep, nullptr);
   }
 }
 

@@ -31,12 +31,13 @@ __global__ __launch_bounds__(64) void k_raycast_wave(SceneDev sc, rlevox_raymap 
                                                      int32_t* __restrict__ clip,
                                                      unsigned long long* counts) {
   __shared__ uint4 rcache[64 * kRunCache / 8];
+  __shared__ int sweep[64];
   float* dda = reinterpret_cast<float*>(rcache);  // see kDdaLds
   extern __shared__ uint32_t lds[];
   const int x = blockIdx.x;
   if (x >= rm.map_line_count) return;
   render_line_wave<COUNT>(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs, clip + 2 * x, lds, dda,
-                          reinterpret_cast<uint16_t*>(rcache), counts);
+                          reinterpret_cast<uint16_t*>(rcache), sweep, counts);
 }
 
 __global__ __launch_bounds__(256) void k_compose(rlevox_raymap rm, const uint32_t* __restrict__ raybuf,

@@ -91,6 +91,19 @@ __device__ inline int rank_in(const float* arr, float v, int guess) {
   return j;
 }
 
+// Exclusive prefix maximum over the lanes of the wavefront (lane 0 gets
+// `init`).
+__device__ inline int prefix_max_excl(int v, int init, int lane) {
+  int incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int o = __shfl_up(incl, off);
+    if (lane >= off) incl = max(incl, o);
+  }
+  const int prev = __shfl_up(incl, 1);
+  return lane == 0 ? init : max(init, prev);
+}
+
 // ------------------------------------------------------------ span pixels
 
 // Sets the bitmap bits of rows base..base+63 given by mask m.  Lane 0 issues
@@ -362,7 +375,7 @@ __device__ inline void dda_gather(const DdaWalk& w, DdaChunk& c, float vpx, floa
 template <bool COUNT>
 __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_raymap& rm,
                                         uint32_t* __restrict__ row, int32_t* __restrict__ clip,
-                                        uint32_t* bits, float* dl, uint16_t* rc,
+                                        uint32_t* bits, float* dl, uint16_t* rc, int* sweep_lds,
                                         unsigned long long* counts) {
   const int lane = threadIdx.x & 63;
   const int rs = rm.rs;
@@ -676,6 +689,79 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
       const bool culled = top <= pz * (float)ymin;
       const bool act = valid && lane >= cur_step && !culled && F > ymin;
       const uint64_t am = uni64(__ballot(act));
+      if (!COUNT && (am & (am - 1))) {
+        // ---- horizon sweep: a run of acting steps applied at once.  The
+        // typical step draws its first element's span from the horizon up
+        // (s1 <= ymin), leaves the top clip alone (F < ymax) and breaks at
+        // its next element (F2 <= F): its only effects are ymin = F and the
+        // rows [ymin, F) drawn.  The horizon before step t is then the prefix
+        // maximum M_t of the earlier actors' F.  Each lane checks that, under
+        // M_t, it acts exactly when the speculation says it does and that
+        // its step is of that simple kind; the steps before the first lane
+        // failing a check, whose spans together stay within 64 rows that are
+        // all clear in the bitmap (row F included, where the horizon stops),
+        // are applied here exactly as the ordered replay would apply them.
+        const int Fs = act ? F : INT32_MIN;
+        const int M = prefix_max_excl(Fs, ymin, lane);
+        const bool spec_actor = act && F > M;
+        const bool true_actor = valid && lane >= cur_step && !(top <= pz * (float)M) && F > M;
+        const bool simple = cand && drawF && s1F <= M && F < ymax && F2 <= F && F - ymin <= 63;
+        const bool ok = spec_actor == true_actor && (!spec_actor || simple);
+        const uint64_t bad = uni64(__ballot(lane >= cur_step && !ok));
+        const int limit = uni(bad ? (int)__builtin_ctzll(bad) : 64);
+        const uint64_t acc = uni64(__ballot(spec_actor && lane < limit));
+        if (acc) {
+          const int last = 63 - (int)__builtin_clzll(acc);
+          const int fend = uni(readlane_i(F, last));
+          // rows ymin..fend (fend < ymin + 64) must all be clear
+          const bool rclear = lane > fend - ymin || !lds_bit(bits, ymin + lane);
+          if (uni64(__ballot(rclear)) == ~0ull) {
+            // row r = ymin + j belongs to the accepted step with the highest
+            // start row <= j (starts are marked in LDS)
+            sweep_lds[lane] = -1;
+            wave_sync();
+            if ((acc >> lane) & 1) sweep_lds[M - ymin] = lane;
+            wave_sync();
+            const int mark = sweep_lds[lane];
+            const uint64_t smask = uni64(__ballot(mark >= 0));
+            const int nrows = fend - ymin;
+            const bool mine = lane < nrows;
+            const uint64_t upto = lane >= 63 ? ~0ull : ((2ull << lane) - 1ull);
+            const int srow = 63 - (int)__builtin_clzll((smask & upto) | 1ull);
+            const int a0 = __shfl(mark, srow);
+            const int a = mine ? a0 : 0;
+            const float s1r = __shfl(s1rF, a), u1z = __shfl(u1zF, a), u2dz = __shfl(u2dzF, a);
+            const float onez1 = __shfl(onez1F, a), onedz2 = __shfl(onedz2F, a);
+            const float thi = __shfl(texF, a) - 1.0f;
+            const float tlo = (float)__shfl(textureF, a);
+            const uint32_t aofs = (uint32_t)__shfl((int)ent.x, a);
+            const uint32_t alen = (uint32_t)__shfl((int)ent.y, a) & 0xffffu;
+            if (mine) {
+              const int y0 = ymin + srow;
+              const float mult = (float)(y0 + 1) - s1r;
+              float uz = u1z + u2dz * mult, oz = onez1 + onedz2 * mult;
+              for (int i = srow; i < lane; i++) {
+                uz += u2dz;
+                oz += onedz2;
+              }
+              const float fu = fminf(fmaxf(uz / oz, tlo), thi);
+              const uint32_t u = (uint32_t)d_f2i(fu);
+              const uint32_t depth = (uint32_t)d_f2i(1.0f / oz) & 0xfffeu;
+              const uint64_t wofs = (uint64_t)aofs << shift;
+              const int tdelta = (int)(wofs & 7u);
+              const int cw = 2 + (int)alen + tdelta + (int)u;
+              const uint32_t col = cw < kRunCache ? rc[a * kRunCache + cw]
+                                                  : slab[2 + wofs + alen + u];
+              row[ymin + lane] = col + (depth << 16);
+            }
+            bits_or(bits, ymin, nrows >= 64 ? ~0ull : ((1ull << nrows) - 1ull), lane);
+            wave_sync();
+            ymin = fend;
+            cur_step = limit;
+            continue;
+          }
+        }
+      }
       const int t = uni(am ? (int)__builtin_ctzll(am) : L);
       if (COUNT) {
         // no-op steps [cur_step, t): pointer loads and elements walked to the break
