@@ -67,7 +67,7 @@ constexpr int kRunCache = 40;
 // LDS floats per wavefront used by the DDA chunk.  The kernels place them in
 // the run cache's memory: the DDA of chunk c+1 runs after chunk c's replay
 // has finished with the cache and before chunk c+1's cache is stored.
-constexpr int kDdaLds = 7 * 64;
+constexpr int kDdaLds = 6 * 68 + 64;
 static_assert(kDdaLds * 4 <= 64 * kRunCache * 2, "DDA scratch must fit in the run cache");
 
 // First y in [from, to) whose bit is clear, or `to` (from <= to).
@@ -279,54 +279,66 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
   // seen k additions whatever the interleaving).  Lane k then reads term k.
   // The walk's order of x and z steps is the merge of the two distance
   // sequences (ties go to x: the walk takes z only when strictly closer).
+  // Layout: six sequences of 65 terms at a stride of 68 floats (the writers,
+  // lanes 0..5, hit different LDS banks), then 64 merge slots.
+  constexpr int kSeq = 68;
   float* sA = dl;
-  float* sB = dl + 64;
-  float* sX0 = dl + 128;
-  float* sY0 = dl + 192;
-  float* sX1 = dl + 256;
-  float* sY1 = dl + 320;
-  int* slot = reinterpret_cast<int*>(dl + 384);
+  float* sB = dl + kSeq;
+  float* sX0 = dl + 2 * kSeq;
+  float* sY0 = dl + 3 * kSeq;
+  float* sX1 = dl + 4 * kSeq;
+  float* sY1 = dl + 5 * kSeq;
+  int* slot = reinterpret_cast<int*>(dl + 6 * kSeq);
   {
     if (lane < 6) {
       float t = w.dd0, g = w.gd0;
-      if (lane == 1) {
-        t = w.dd1;
-        g = w.gd1;
-      } else if (lane == 2) {
-        t = w.i0x;
-        g = w.g0x;
-      } else if (lane == 3) {
-        t = w.i0y;
-        g = w.g0y;
-      } else if (lane == 4) {
-        t = w.i1x;
-        g = w.g1x;
-      } else if (lane == 5) {
-        t = w.i1y;
-        g = w.g1y;
-      }
-      float* dst = dl + lane * 64;
+      t = lane == 1 ? w.dd1 : t;
+      g = lane == 1 ? w.gd1 : g;
+      t = lane == 2 ? w.i0x : t;
+      g = lane == 2 ? w.g0x : g;
+      t = lane == 3 ? w.i0y : t;
+      g = lane == 3 ? w.g0y : g;
+      t = lane == 4 ? w.i1x : t;
+      g = lane == 4 ? w.g1x : g;
+      t = lane == 5 ? w.i1y : t;
+      g = lane == 5 ? w.g1y : g;
+      float* dst = dl + lane * kSeq;
       dst[0] = t;
 #pragma unroll
-      for (int k = 1; k < 64; k++) {
+      for (int k = 1; k <= 64; k++) {  // term 64 is the state after a full chunk
         t = t + g;
         dst[k] = t;
       }
     }
     wave_sync();
     const float ak = sA[lane], bk = sB[lane];
-    // merge ranks: the sequences grow by about gd per term
-    const int gx = d_f2i(ceilf(__fdividef(ak - w.dd1, w.gd1)));
-    const int gz = d_f2i(ceilf(__fdividef(bk - w.dd0, w.gd0)));
-    const int rx = lane + rank_in<false>(sB, ak, gx);
-    const int rz = lane + rank_in<true>(sA, bk, gz);
+    // Merge ranks: rx = lane + #{j : sB[j] < ak}, rz = lane + #{j : sA[j] <= bk}.
+    // The sequences grow by about gd per term, so a guess g is off by at most
+    // one in practice; three probes around it settle the rank exactly
+    // (the sequences are sorted), else the lane searches.
+    const float qx = (ak - w.dd1) * __builtin_amdgcn_rcpf(w.gd1);
+    const float qz = (bk - w.dd0) * __builtin_amdgcn_rcpf(w.gd0);
+    const int gx = (int)fminf(fmaxf(ceilf(qx), 0.0f), 64.0f);
+    const int gz = (int)fminf(fmaxf(ceilf(qz), 0.0f), 64.0f);
+    const float bm = sB[max(gx - 1, 0)], b0 = sB[gx], bp = sB[min(gx + 1, 64)];
+    const float am = sA[max(gz - 1, 0)], a0 = sA[gz], ap = sA[min(gz + 1, 64)];
+    const bool okx = (gx == 0 || bm < ak) && (gx >= 63 || !(bp < ak));
+    const bool okz = (gz == 0 || !(bk < am)) && (gz >= 63 || bk < ap);
+    int cx = gx + ((gx < 64 && b0 < ak) ? 1 : 0);
+    int cz = gz + ((gz < 64 && !(bk < a0)) ? 1 : 0);
+    if (!okx) cx = rank_in<false>(sB, ak, gx);
+    if (!okz) cz = rank_in<true>(sA, bk, gz);
+    const int rx = lane + cx;
+    const int rz = lane + cz;
     if (rx < L) slot[rx] = lane;
     if (rz < L) slot[rz] = 64 + lane;
     const int nx = (int)__popcll(__ballot(rx < L));
     const int nz = L - nx;
     wave_sync();
+    const int cc = lane < L ? slot[lane] : 0;
+    const int p = (lane < L && lane > 0) ? slot[lane - 1] : 0;
+    const int cl = uni(slot[L - 1]);
     if (lane < L) {
-      const int cc = slot[lane];
       c.dn = cc < 64 ? sA[cc] : sB[cc - 64];
       if (lane == 0) {
         c.db = w.dnow;
@@ -334,24 +346,22 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
         c.pby = w.py;
         c.ib = w.idx;
       } else {
-        const int p = slot[lane - 1];
         c.db = p < 64 ? sA[p] : sB[p - 64];
         c.pbx = p < 64 ? sX0[p] : sX1[p - 64];
         c.pby = p < 64 ? sY0[p] : sY1[p - 64];
         c.ib = p < 64 ? 0 : 1;
       }
     }
-    const int cl = uni(slot[L - 1]);
     w.dnow = uni_f(cl < 64 ? sA[cl] : sB[cl - 64]);
     w.px = uni_f(cl < 64 ? sX0[cl] : sX1[cl - 64]);
     w.py = uni_f(cl < 64 ? sY0[cl] : sY1[cl - 64]);
     w.idx = cl < 64 ? 0 : 1;
-    w.dd0 = uni_f(nx < 64 ? sA[nx] : sA[63] + w.gd0);
-    w.i0x = uni_f(nx < 64 ? sX0[nx] : sX0[63] + w.g0x);
-    w.i0y = uni_f(nx < 64 ? sY0[nx] : sY0[63] + w.g0y);
-    w.dd1 = uni_f(nz < 64 ? sB[nz] : sB[63] + w.gd1);
-    w.i1x = uni_f(nz < 64 ? sX1[nz] : sX1[63] + w.g1x);
-    w.i1y = uni_f(nz < 64 ? sY1[nz] : sY1[63] + w.g1y);
+    w.dd0 = uni_f(sA[nx]);
+    w.i0x = uni_f(sX0[nx]);
+    w.i0y = uni_f(sY0[nx]);
+    w.dd1 = uni_f(sB[nz]);
+    w.i1x = uni_f(sX1[nz]);
+    w.i1y = uni_f(sY1[nz]);
     wave_sync();
   }
   w.z += (float)L * w.dz;
