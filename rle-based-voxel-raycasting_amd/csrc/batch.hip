@@ -83,18 +83,18 @@ __global__ __launch_bounds__(256) void k_compose_batch(const BatchHeader* __rest
 
 __global__ __launch_bounds__(256) void k_gapfill_batch(const BatchHeader* __restrict__ h,
                                                        const ViewDesc* __restrict__ views) {
+  __shared__ GapTile gt;
   const int64_t total = h->total_tiles;
   const int nv = h->nviews;
   for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
     const int v = find_view<true>(views, nv, tile);
     const ViewDesc& vd = views[v];
-    if (!(vd.post & RLEVOX_POST_GAPFILL)) continue;
-    const int W = vd.rm.W, H = vd.rm.H;
+    if (!(vd.post & RLEVOX_POST_GAPFILL)) continue;  // uniform over the block
+    const int W = vd.rm.W;
     const int64_t local = tile - vd.tile_begin;
     const int tiles_x = (W + 63) / 64;
-    const int px = (int)(local % tiles_x) * 64 + (threadIdx.x & 63);
-    const int row = (int)(local / tiles_x) * 4 + (threadIdx.x >> 6);
-    if (px < W && row < H) vd.colour[(size_t)row * W + px] = gapfill_pixel(W, H, vd.ctmp, vd.depth, px, row);
+    gapfill_tile(W, vd.rm.H, vd.ctmp, vd.depth, vd.colour, (int)(local % tiles_x) * 64,
+                 (int)(local / tiles_x) * 4, gt);
   }
 }
 

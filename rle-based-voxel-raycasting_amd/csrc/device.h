@@ -498,4 +498,64 @@ __device__ inline uint16_t gapfill_pixel(int W, int H, const uint16_t* __restric
   return c0;
 }
 
+// The same gap-fill for one 64x4 tile of a 256-thread block, with the tile
+// and a kGapHalo-pixel border staged in LDS (coordinates clamped to the
+// image as each tap clamps them).  The ring (offsets up to 10 px) and every
+// 5x5 tap set of radius up to kGapHalo px read LDS; pixels nearer than that
+// (larger radii) read their taps from memory.  Call from all 256 threads.
+constexpr int kGapHalo = 12;
+constexpr int kGapTW = 64 + 2 * kGapHalo, kGapTH = 4 + 2 * kGapHalo;
+struct GapTile {
+  int32_t d[kGapTH][kGapTW];
+  uint16_t c[kGapTH][kGapTW];
+};
+
+__device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ cin,
+                                    const int32_t* __restrict__ dep, uint16_t* __restrict__ cout,
+                                    int x0, int y0, GapTile& t) {
+  for (int i = threadIdx.x; i < kGapTW * kGapTH; i += 256) {
+    const int ty = i / kGapTW, tx = i - ty * kGapTW;
+    const int gx = min(max(x0 - kGapHalo + tx, 0), W - 1);
+    const int gy = min(max(y0 - kGapHalo + ty, 0), H - 1);
+    const size_t g = (size_t)gy * W + gx;
+    t.d[ty][tx] = dep[g];
+    t.c[ty][tx] = cin[g];
+  }
+  __syncthreads();
+  const int lx = (threadIdx.x & 63) + kGapHalo, ly = (threadIdx.x >> 6) + kGapHalo;
+  const int x = x0 + (threadIdx.x & 63), prow = y0 + (threadIdx.x >> 6);
+  if (x < W && prow < H) {
+    const uint16_t c0 = t.c[ly][lx];
+    int dmin = t.d[ly][lx];
+    for (int k = 0; k < 7; k++) dmin = min(dmin, t.d[ly - kRingDy[k]][lx + kRingDx[k]]);
+    uint16_t out = c0;
+    if (dmin < RLEVOX_SKY_DEPTH) {
+      const int dm = max(dmin, 1);
+      int r_q8 = (int)(78720.0f / (float)dm);  // floor(78720 / dm), corrected below
+      if (r_q8 * dm > 78720) r_q8--;
+      if (r_q8 >= 42) {
+        if (floor_div_pos(640 - 5 * r_q8, 1280) >= -kGapHalo) {
+          int n = 1, sum = c0 & 0xff;
+          for (int a = 0; a < 5; a++) {
+            const int ox = floor_div_pos((2 * a - 5) * r_q8 + 640, 1280);
+            for (int b = 0; b < 5; b++) {
+              const int oy = floor_div_pos((2 * b - 5) * r_q8 + 640, 1280);
+              const int d = t.d[ly - oy][lx + ox];
+              if (d < RLEVOX_SKY_DEPTH && (int64_t)10 * dmin >= (int64_t)7 * d) {
+                n++;
+                sum += t.c[ly - oy][lx + ox] & 0xff;
+              }
+            }
+          }
+          if (n > 6) out = (uint16_t)((c0 & 0xff00) | ((sum + n / 2) / n));
+        } else {
+          out = gapfill_pixel(W, H, cin, dep, x, prow);
+        }
+      }
+    }
+    cout[(size_t)prow * W + x] = out;
+  }
+  __syncthreads();  // the block's next tile reuses the LDS
+}
+
 }  // namespace rlevox

@@ -57,10 +57,8 @@ __global__ __launch_bounds__(256) void k_compose(rlevox_raymap rm, const uint32_
 __global__ __launch_bounds__(256) void k_gapfill(int W, int H, const uint16_t* __restrict__ cin,
                                                  const int32_t* __restrict__ dep,
                                                  uint16_t* __restrict__ cout) {
-  const int px = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int row = blockIdx.y * 4 + (threadIdx.x >> 6);
-  if (px >= W || row >= H) return;
-  cout[(size_t)row * W + px] = gapfill_pixel(W, H, cin, dep, px, row);
+  __shared__ GapTile tile;
+  gapfill_tile(W, H, cin, dep, cout, blockIdx.x * 64, blockIdx.y * 4, tile);
 }
 
 hipError_t launch_raycast_plain(const SceneDev& sc, const rlevox_raymap& rm, uint32_t* raybuf,
