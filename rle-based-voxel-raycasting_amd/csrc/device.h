@@ -534,16 +534,25 @@ __device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ c
       int r_q8 = (int)(78720.0f / (float)dm);  // floor(78720 / dm), corrected below
       if (r_q8 * dm > 78720) r_q8--;
       if (r_q8 >= 42) {
-        if (floor_div_pos(640 - 5 * r_q8, 1280) >= -kGapHalo) {
+        // tap offsets floor(((2a-5) r + 640) / 1280); the numerators stay
+        // above -655360, so a bias of 512*1280 keeps the division unsigned
+        int o[5];
+#pragma unroll
+        for (int a = 0; a < 5; a++)
+          o[a] = (int)((uint32_t)((2 * a - 5) * r_q8 + 640 + 655360) / 1280u) - 512;
+        if (o[0] >= -kGapHalo) {
+          // r_q8 >= 42 means dmin <= 1874: a tap qualifies when
+          // 10*dmin >= 7*d, i.e. d <= floor(10*dmin/7) (which is below the sky depth)
           int n = 1, sum = c0 & 0xff;
+          const int dlim = (10 * dmin) / 7;
+#pragma unroll
           for (int a = 0; a < 5; a++) {
-            const int ox = floor_div_pos((2 * a - 5) * r_q8 + 640, 1280);
+#pragma unroll
             for (int b = 0; b < 5; b++) {
-              const int oy = floor_div_pos((2 * b - 5) * r_q8 + 640, 1280);
-              const int d = t.d[ly - oy][lx + ox];
-              if (d < RLEVOX_SKY_DEPTH && (int64_t)10 * dmin >= (int64_t)7 * d) {
+              const int d = t.d[ly - o[b]][lx + o[a]];
+              if (d <= dlim) {
                 n++;
-                sum += t.c[ly - oy][lx + ox] & 0xff;
+                sum += t.c[ly - o[b]][lx + o[a]] & 0xff;
               }
             }
           }
