@@ -332,12 +332,19 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
     const int rz = lane + cz;
     if (rx < L) slot[rx] = lane;
     if (rz < L) slot[rz] = 64 + lane;
-    const int nx = (int)__popcll(__ballot(rx < L));
+    const int nx = uni((int)__popcll(__ballot(rx < L)));
     const int nz = L - nx;
+    // The walk's state after the chunk, read in one round trip: the next
+    // terms (index nx of the x sequences, nz of the z sequences) and the last
+    // step, which is the later of x-term nx-1 and z-term nz-1 (a z-term
+    // follows an equal x-term).
+    const float nA = sA[nx], nX0 = sX0[nx], nY0 = sY0[nx];
+    const float nB = sB[nz], nX1 = sX1[nz], nY1 = sY1[nz];
+    const float lA = sA[max(nx - 1, 0)], lX0 = sX0[max(nx - 1, 0)], lY0 = sY0[max(nx - 1, 0)];
+    const float lB = sB[max(nz - 1, 0)], lX1 = sX1[max(nz - 1, 0)], lY1 = sY1[max(nz - 1, 0)];
     wave_sync();
     const int cc = lane < L ? slot[lane] : 0;
     const int p = (lane < L && lane > 0) ? slot[lane - 1] : 0;
-    const int cl = uni(slot[L - 1]);
     if (lane < L) {
       c.dn = cc < 64 ? sA[cc] : sB[cc - 64];
       if (lane == 0) {
@@ -352,16 +359,17 @@ __device__ inline bool dda_next_chunk(DdaWalk& w, DdaChunk& c, const SceneDev& s
         c.ib = p < 64 ? 0 : 1;
       }
     }
-    w.dnow = uni_f(cl < 64 ? sA[cl] : sB[cl - 64]);
-    w.px = uni_f(cl < 64 ? sX0[cl] : sX1[cl - 64]);
-    w.py = uni_f(cl < 64 ? sY0[cl] : sY1[cl - 64]);
-    w.idx = cl < 64 ? 0 : 1;
-    w.dd0 = uni_f(sA[nx]);
-    w.i0x = uni_f(sX0[nx]);
-    w.i0y = uni_f(sY0[nx]);
-    w.dd1 = uni_f(sB[nz]);
-    w.i1x = uni_f(sX1[nz]);
-    w.i1y = uni_f(sY1[nz]);
+    const bool last_z = nz > 0 && (nx == 0 || !(lB < lA));
+    w.dnow = uni_f(last_z ? lB : lA);
+    w.px = uni_f(last_z ? lX1 : lX0);
+    w.py = uni_f(last_z ? lY1 : lY0);
+    w.idx = last_z ? 1 : 0;
+    w.dd0 = uni_f(nA);
+    w.i0x = uni_f(nX0);
+    w.i0y = uni_f(nY0);
+    w.dd1 = uni_f(nB);
+    w.i1x = uni_f(nX1);
+    w.i1y = uni_f(nY1);
     wave_sync();
   }
   w.z += (float)L * w.dz;
