@@ -538,11 +538,11 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
   // of chunk c+1 is in flight.  The walk does
   // not depend on the replay (only the line's end does), so a chunk
   // computed past the end of the line is simply dropped.
-  DdaChunk cur;
-  bool have = dda_next_chunk(wk, cur, sc, zfar, dl, lane);
-  if (have) dda_gather(wk, cur, vpx, vpz, fix_xf, fix_yf, lane);
-  while (have) {
-    if (ymin >= ymax) break;
+  // The chunk loop is unrolled twice over two register sets (chunk c in one,
+  // c+1 in the other), so the gathered entries never have to be copied (a
+  // copy would wait for the load).
+  auto chunk_step = [&](const DdaChunk& cur, DdaChunk& nxt, bool& have_next) -> bool {
+    if (ymin >= ymax) return false;
     const int L = cur.L, mip = cur.mip, shift = cur.shift;
     const uint16_t* slab = cur.slab;
     const uint2 ent = cur.ent;
@@ -626,8 +626,7 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     }
 
     // ---- next chunk's DDA (the run cache loads are in flight meanwhile)
-    DdaChunk nxt;
-    const bool have_next = dda_next_chunk(wk, nxt, sc, zfar, dl, lane);
+    have_next = dda_next_chunk(wk, nxt, sc, zfar, dl, lane);
 
     if (cand) {
       uint4* dst = reinterpret_cast<uint4*>(rc + lane * kRunCache);
@@ -933,9 +932,14 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
       }
       cur_step = t + 1;
     }
-    if (done) break;
-    cur = nxt;
-    have = have_next;
+    return !done;
+  };
+  DdaChunk ca, cb;
+  bool ha = dda_next_chunk(wk, ca, sc, zfar, dl, lane), hb = false;
+  if (ha) dda_gather(wk, ca, vpx, vpz, fix_xf, fix_yf, lane);
+  for (;;) {
+    if (!ha || !chunk_step(ca, cb, hb)) break;
+    if (!hb || !chunk_step(cb, ca, ha)) break;
   }
   if (COUNT && lane == 0) {
     atomicAdd(counts + 0, 1ull);
