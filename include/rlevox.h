@@ -169,7 +169,10 @@ typedef struct {
   int32_t lines_per_wave;  /* 1 = one wavefront walks each line (lanes take
                               consecutive DDA steps); 16, 32 or 64 = one lane
                               per line */
-  int32_t waves_per_block; /* 1..4 */
+  int32_t waves_per_block; /* 1..4; with lines_per_wave 1: 1 = one wavefront
+                              both walks and draws a line, 2..4 = two
+                              wavefronts per line, one walking the DDA ahead
+                              of the one drawing (plain renders only) */
   int32_t persistent_waves; /* wavefronts per CU of the persistent batch kernel
                                (0 = default 16, else 4..32) */
   int32_t order;           /* 0 = view order, 1 = longest views first */

@@ -99,9 +99,11 @@ struct rlevox_ctx {
   rlevox::Scratch raybuf, clip, ctmp, counts;
   rlevox::BatchSlot batch;
   rlevox_tuning tuning{1, 1, 0, 1};
-  // 1 selects the wavefront-per-line kernel; otherwise lanes per block.
+  // 1 selects the wavefront-per-line kernel, 2 its two-wavefront form;
+  // otherwise lanes per block (16 or more).
   int tuning_block() const {
-    return tuning.lines_per_wave == 1 ? 1 : tuning.lines_per_wave * tuning.waves_per_block;
+    if (tuning.lines_per_wave == 1) return tuning.waves_per_block >= 2 ? 2 : 1;
+    return tuning.lines_per_wave * tuning.waves_per_block;
   }
   void release_all() {
     raybuf.release();

@@ -40,6 +40,22 @@ __global__ __launch_bounds__(64) void k_raycast_wave(SceneDev sc, rlevox_raymap 
                           reinterpret_cast<uint16_t*>(rcache), sweep, counts);
 }
 
+// Two wavefronts (= one block) per ray line: wavefront 0 walks the DDA and
+// gathers the pointer map, wavefront 1 projects, caches and draws.
+__global__ __launch_bounds__(128) void k_raycast_wave2(SceneDev sc, rlevox_raymap rm,
+                                                       uint32_t* __restrict__ raybuf,
+                                                       int32_t* __restrict__ clip) {
+  __shared__ uint4 rcache[64 * kRunCache / 8];
+  __shared__ int sweep[64];
+  __shared__ float dda[kDdaLds];
+  __shared__ ChunkRing ring;
+  extern __shared__ uint32_t lds[];
+  const int x = blockIdx.x;
+  if (x >= rm.map_line_count) return;
+  render_line_wave<false, true>(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs, clip + 2 * x, lds,
+                                dda, reinterpret_cast<uint16_t*>(rcache), sweep, nullptr, &ring);
+}
+
 __global__ __launch_bounds__(256) void k_compose(rlevox_raymap rm, const uint32_t* __restrict__ raybuf,
                                                  const int32_t* __restrict__ clip,
                                                  uint16_t* __restrict__ col,
@@ -66,6 +82,14 @@ hipError_t launch_raycast_plain(const SceneDev& sc, const rlevox_raymap& rm, uin
                                 unsigned long long* counts) {
   const int lines = rm.map_line_count;
   if (lines <= 0) return hipSuccess;
+  if (block == 2) {  // two wavefronts per line (counting uses one)
+    const size_t lds = (size_t)bitmap_words(rm.rs) * sizeof(uint32_t);
+    if (!counts) {
+      hipLaunchKernelGGL(k_raycast_wave2, dim3(lines), dim3(128), lds, s, sc, rm, raybuf, clip);
+      return hipGetLastError();
+    }
+    block = 1;
+  }
   if (block == 1) {  // one wavefront per line
     const size_t lds = (size_t)bitmap_words(rm.rs) * sizeof(uint32_t);
     if (counts)

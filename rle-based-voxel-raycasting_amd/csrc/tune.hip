@@ -13,10 +13,10 @@ using namespace rlevox;
 
 namespace {
 
-// Candidates: one wavefront per line (lanes take consecutive DDA steps), and
-// one lane per line in blocks of 16..256 lanes.
+// Candidates: one or two wavefronts per line (lanes take consecutive DDA
+// steps), and one lane per line in blocks of 16..256 lanes.
 const rlevox_tuning kCandidates[] = {
-    {1, 1, 0, 1},  {64, 1, 0, 1}, {64, 2, 0, 1}, {64, 4, 0, 1},
+    {1, 1, 0, 1},  {1, 2, 0, 1},  {64, 1, 0, 1}, {64, 2, 0, 1}, {64, 4, 0, 1},
     {32, 1, 0, 1}, {32, 2, 0, 1}, {16, 1, 0, 1}, {16, 4, 0, 1},
 };
 

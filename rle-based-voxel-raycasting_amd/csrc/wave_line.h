@@ -27,6 +27,8 @@
 //     ballot of the drawn pixels.
 #pragma once
 
+#include <type_traits>
+
 #include "device.h"
 
 namespace rlevox {
@@ -390,15 +392,61 @@ __device__ inline void dda_gather(const DdaWalk& w, DdaChunk& c, float vpx, floa
   if (lane < c.L) c.ent = w.map[vx + vz * w.gridx];
 }
 
-template <bool COUNT>
+// Chunks handed from the producer wavefront (DDA + pointer gather) to the
+// consumer wavefront (projection, run cache, replay) of a two-wavefront
+// line: three slots used round robin.
+struct ChunkRing {
+  float db[3][64], dn[3][64];
+  uint2 ent[3][64];
+  const uint16_t* slab[3];
+  int L[3], mip[3], shift[3], have[3];
+  int stop[3];  // written by the consumer in the round that uses the slot
+};
+
+// The fields the consumer reads: distances, level and the gathered entry.
+__device__ inline void ring_put_dda(ChunkRing& r, int s, const DdaChunk& c, bool have, int lane) {
+  if (have) {
+    r.db[s][lane] = c.db;
+    r.dn[s][lane] = c.dn;
+  }
+  if (lane == 0) {
+    r.slab[s] = c.slab;
+    r.L[s] = have ? c.L : 0;
+    r.mip[s] = c.mip;
+    r.shift[s] = c.shift;
+    r.have[s] = have ? 1 : 0;
+  }
+}
+
+__device__ inline void ring_put_ent(ChunkRing& r, int s, const DdaChunk& c, int lane) {
+  r.ent[s][lane] = c.ent;  // waits for the gather
+}
+
+__device__ inline void ring_get(const ChunkRing& r, int s, DdaChunk& c, int lane) {
+  c.L = uni(r.L[s]);
+  c.mip = uni(r.mip[s]);
+  c.shift = uni(r.shift[s]);
+  c.slab = (const uint16_t*)uni64((uint64_t)r.slab[s]);
+  c.db = r.db[s][lane];
+  c.dn = r.dn[s][lane];
+  c.ent = r.ent[s][lane];
+}
+
+// TWO: the line is walked by a 128-thread block, wavefront 0 producing the
+// DDA chunks (and their pointer gathers) two chunks ahead of wavefront 1,
+// which consumes them; the two halves of the work overlap.  Otherwise one
+// wavefront does both, software-pipelined.
+template <bool COUNT, bool TWO = false>
 __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_raymap& rm,
                                         uint32_t* __restrict__ row, int32_t* __restrict__ clip,
                                         uint32_t* bits, float* dl, uint16_t* rc, int* sweep_lds,
-                                        unsigned long long* counts) {
+                                        unsigned long long* counts, ChunkRing* ring = nullptr) {
+  static_assert(!(COUNT && TWO), "the counting variant walks with one wavefront");
   const int lane = threadIdx.x & 63;
+  const bool producer = TWO && threadIdx.x < 64;  // wavefront 0 of a two-wavefront line
   const int rs = rm.rs;
   const float half = (float)(rs / 2);
-  if (lane == 0) {
+  if (lane == 0 && !producer) {
     clip[0] = 0;
     clip[1] = -1;
   }
@@ -474,14 +522,16 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
   if (ymin >= ymax) return;
 
   // ---- clear the range and the bitmap (Cuda_Render.h:255-264), coalesced
-  for (int n = ymin + lane; n <= ymax; n += 64) row[n] = RLEVOX_CLEAR_TEXEL;
-  const int words = bitmap_words(rs);
-  for (int w = lane; w < words; w += 64) bits[w] = 0;
-  if (lane == 0) {
-    clip[0] = ymin;
-    clip[1] = ymax;
+  if (!producer) {
+    for (int n = ymin + lane; n <= ymax; n += 64) row[n] = RLEVOX_CLEAR_TEXEL;
+    const int words = bitmap_words(rs);
+    for (int w = lane; w < words; w += 64) bits[w] = 0;
+    if (lane == 0) {
+      clip[0] = ymin;
+      clip[1] = ymax;
+    }
+    wave_sync();
   }
-  wave_sync();
   LineCounts cnt;
   if (COUNT) cnt.cleared = (uint64_t)(ymax - ymin + 1);
 
@@ -541,7 +591,7 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
   // The chunk loop is unrolled twice over two register sets (chunk c in one,
   // c+1 in the other), so the gathered entries never have to be copied (a
   // copy would wait for the load).
-  auto chunk_step = [&](const DdaChunk& cur, DdaChunk& nxt, bool& have_next) -> bool {
+  auto chunk_step = [&](auto produce, const DdaChunk& cur, DdaChunk& nxt, bool& have_next) -> bool {
     if (ymin >= ymax) return false;
     const int L = cur.L, mip = cur.mip, shift = cur.shift;
     const uint16_t* slab = cur.slab;
@@ -626,7 +676,7 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     }
 
     // ---- next chunk's DDA (the run cache loads are in flight meanwhile)
-    have_next = dda_next_chunk(wk, nxt, sc, zfar, dl, lane);
+    if constexpr (decltype(produce)::value) have_next = dda_next_chunk(wk, nxt, sc, zfar, dl, lane);
 
     if (cand) {
       uint4* dst = reinterpret_cast<uint4*>(rc + lane * kRunCache);
@@ -637,7 +687,9 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
       dst[4] = rc4;
     }
     // ---- next chunk's pointer gather: in flight during this chunk's replay
-    if (have_next) dda_gather(wk, nxt, vpx, vpz, fix_xf, fix_yf, lane);
+    if constexpr (decltype(produce)::value) {
+      if (have_next) dda_gather(wk, nxt, vpx, vpz, fix_xf, fix_yf, lane);
+    }
     // first element's lower end and span parameters (Cuda_Render.h:554-660)
     int s1F = 0;
     bool drawF = false;
@@ -934,12 +986,74 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     }
     return !done;
   };
-  DdaChunk ca, cb;
-  bool ha = dda_next_chunk(wk, ca, sc, zfar, dl, lane), hb = false;
-  if (ha) dda_gather(wk, ca, vpx, vpz, fix_xf, fix_yf, lane);
-  for (;;) {
-    if (!ha || !chunk_step(ca, cb, hb)) break;
-    if (!hb || !chunk_step(cb, ca, ha)) break;
+  if constexpr (!TWO) {
+    DdaChunk ca, cb;
+    bool ha = dda_next_chunk(wk, ca, sc, zfar, dl, lane), hb = false;
+    if (ha) dda_gather(wk, ca, vpx, vpz, fix_xf, fix_yf, lane);
+    const std::true_type yes;
+    for (;;) {
+      if (!ha || !chunk_step(yes, ca, cb, hb)) break;
+      if (!hb || !chunk_step(yes, cb, ca, ha)) break;
+    }
+  } else {
+    // Round i: slot i%3 holds chunk i complete, slot (i+1)%3 chunk i+1's DDA
+    // (its gather in flight in the producer).  The producer walks chunk i+2
+    // and stores chunk i+1's entries while the consumer processes chunk i;
+    // a block barrier ends the round.  Both wavefronts read the same flags
+    // after each barrier (have[i%3] last written two rounds before, stop[i%3]
+    // in this round), so they run the same rounds and the same barriers.
+    ChunkRing& r = *ring;
+    if (producer) {
+      DdaChunk p0, p1;
+      const bool h0 = dda_next_chunk(wk, p0, sc, zfar, dl, lane);
+      ring_put_dda(r, 0, p0, h0, lane);
+      if (h0) dda_gather(wk, p0, vpx, vpz, fix_xf, fix_yf, lane);
+      const bool h1 = h0 && dda_next_chunk(wk, p1, sc, zfar, dl, lane);
+      ring_put_dda(r, 1, p1, h1, lane);
+      if (h0) ring_put_ent(r, 0, p0, lane);
+      if (h1) dda_gather(wk, p1, vpx, vpz, fix_xf, fix_yf, lane);
+      __syncthreads();
+      // two register sets: `pend` holds the chunk whose gather is in flight
+      auto produce_round = [&](int s, DdaChunk& pend, bool& hpend, DdaChunk& nxt) {
+        const int s1 = s == 2 ? 0 : s + 1, s2 = s1 == 2 ? 0 : s1 + 1;
+        const bool h2 = hpend && dda_next_chunk(wk, nxt, sc, zfar, dl, lane);
+        ring_put_dda(r, s2, nxt, h2, lane);
+        if (hpend) ring_put_ent(r, s1, pend, lane);
+        if (h2) dda_gather(wk, nxt, vpx, vpz, fix_xf, fix_yf, lane);
+        hpend = h2;
+      };
+      bool hp = h1;
+      int s = 0;
+      for (;;) {
+        if (!uni(r.have[s])) break;
+        produce_round(s, p1, hp, p0);
+        __syncthreads();
+        if (uni(r.stop[s])) break;
+        s = s == 2 ? 0 : s + 1;
+        if (!uni(r.have[s])) break;
+        produce_round(s, p0, hp, p1);
+        __syncthreads();
+        if (uni(r.stop[s])) break;
+        s = s == 2 ? 0 : s + 1;
+      }
+    } else {
+      __syncthreads();
+      const std::false_type no;
+      DdaChunk cur, unused;
+      bool hu = false;
+      int s = 0;
+      for (;;) {
+        if (!uni(r.have[s])) break;
+        ring_get(r, s, cur, lane);
+        const bool more = chunk_step(no, cur, unused, hu);
+        // A slot's flag is read after this round's barrier and rewritten
+        // three rounds later, so no wavefront can see a later round's value.
+        if (lane == 0) r.stop[s] = more ? 0 : 1;
+        __syncthreads();
+        if (uni(r.stop[s])) break;
+        s = s == 2 ? 0 : s + 1;
+      }
+    }
   }
   if (COUNT && lane == 0) {
     atomicAdd(counts + 0, 1ull);

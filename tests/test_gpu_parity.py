@@ -52,7 +52,7 @@ def assert_same(name, got, want):
 
 # Raycast kernel variants (lines per wavefront): one lane per line in blocks
 # of 64 or 16 lanes, and one wavefront per line.  All must agree bit-exactly.
-KERNELS = [(64, 1), (16, 1), (1, 1)]
+KERNELS = [(64, 1), (16, 1), (1, 1), (1, 2)]
 
 
 def check_view(ctx, oracle, dscene, mips, pose, W, H, mip_distance=0, far_distance=0):
@@ -66,7 +66,7 @@ def check_view(ctx, oracle, dscene, mips, pose, W, H, mip_distance=0, far_distan
     for lpw, wpb in KERNELS:
         ctx.set_tuning(lines_per_wave=lpw, waves_per_block=wpb)
         got = gpu_stages(ctx, dscene, rm)
-        tag = " [lines/wave %d]" % lpw
+        tag = " [lines/wave %d, waves/block %d]" % (lpw, wpb)
         assert_same("line clip" + tag, got["clip"][:n], ref_clip[:n])
         assert_same("ray buffer" + tag, got["raybuf"], ref_rb)
         assert_same("compose colour" + tag, got["compose"], ref_col)
