@@ -44,10 +44,14 @@ def raycast_bytes(c):
             + 4 * (c["cleared"] + c["col_loads"]))
 
 
-def measured_traffic():
+def traffic_profile(kernel):
+    return os.path.join("profiles", "%s_c2_1080p_pmc.txt" % kernel)
+
+
+def measured_traffic(kernel):
     """HBM bytes per raycast launch of this workload, measured by the
     rocprofv3 FETCH_SIZE/WRITE_SIZE passes summarised in profiles/."""
-    path = os.path.join(ROOT, "profiles", "k_raycast_wave_c2_1080p_pmc.txt")
+    path = os.path.join(ROOT, traffic_profile(kernel))
     try:
         for line in open(path):
             if line.startswith("JSON "):
@@ -80,6 +84,8 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=3)
     ap.add_argument("--serve-budget-gb", type=int, default=48)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--waves-per-line", type=int, choices=(1, 2), default=2,
+                    help="wavefronts walking each ray line (2: DDA and drawing overlap)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain copy kernel over MIB MiB (for counter calibration)")
     args = ap.parse_args()
@@ -98,6 +104,8 @@ def main():
     poses = workloads.flythrough(hs.mips[0], frames=100, seed=7)
 
     ctx = rlevox.Context(0)
+    ctx.set_tuning(waves_per_block=args.waves_per_line)
+    kernel = "k_raycast_wave2" if args.waves_per_line == 2 else "k_raycast_wave"
     if args.calibrate_copy:
         nb = args.calibrate_copy << 20
         src, dst = ctx.alloc(nb), ctx.alloc(nb)
@@ -182,10 +190,10 @@ def main():
                    "post_ms_mean": round(float(np.mean(post_ms)), 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "traffic": measured_traffic(), "kernel": "k_raycast_wave",
+                     "traffic": measured_traffic(kernel), "kernel": kernel,
                      "bytes_per_launch": round(mean_bytes),
-                     "traffic_source": "profiles/k_raycast_wave_c2_1080p_pmc.txt (rocprofv3 "
-                                       "FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)"},
+                     "traffic_source": traffic_profile(kernel) + " (rocprofv3 FETCH_SIZE + "
+                                       "WRITE_SIZE, copy-kernel calibrated)"},
     }
 
     if not args.no_batch:

@@ -98,7 +98,7 @@ struct rlevox_ctx {
   std::mutex mu;  // guards the scratch buffers and tuning
   rlevox::Scratch raybuf, clip, ctmp, counts;
   rlevox::BatchSlot batch;
-  rlevox_tuning tuning{1, 1, 0, 1};
+  rlevox_tuning tuning{1, 2, 0, 1};
   // 1 selects the wavefront-per-line kernel, 2 its two-wavefront form;
   // otherwise lanes per block (16 or more).
   int tuning_block() const {
