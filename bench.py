@@ -48,17 +48,25 @@ def traffic_profile(kernel):
     return os.path.join("profiles", "%s_c2_1080p_pmc.txt" % kernel)
 
 
+def issue_profile(kernel):
+    return os.path.join("profiles", "%s_c2_1080p_sq.txt" % kernel)
+
+
+def profile_json(path):
+    """The JSON summary line of a profiles/ file, or {}."""
+    try:
+        for line in open(os.path.join(ROOT, path)):
+            if line.startswith("JSON "):
+                return json.loads(line[5:])
+    except (OSError, ValueError):
+        pass
+    return {}
+
+
 def measured_traffic(kernel):
     """HBM bytes per raycast launch of this workload, measured by the
     rocprofv3 FETCH_SIZE/WRITE_SIZE passes summarised in profiles/."""
-    path = os.path.join(ROOT, traffic_profile(kernel))
-    try:
-        for line in open(path):
-            if line.startswith("JSON "):
-                return json.loads(line[5:])["traffic_bytes"]
-    except (OSError, ValueError, KeyError):
-        pass
-    return None
+    return profile_json(traffic_profile(kernel)).get("traffic_bytes")
 
 
 def post_bytes(W, H):
@@ -79,7 +87,7 @@ def main():
     ap.add_argument("--c3", action="store_true",
                     help="also time 3840x2160 grazing views over a 16384^2 x 1024 map")
     ap.add_argument("--serve-requests", type=int, default=1500)
-    ap.add_argument("--serve-rates", type=int, nargs="*", default=[500, 1000, 2000])
+    ap.add_argument("--serve-rates", type=int, nargs="*", default=[200, 500, 1000, 2000])
     ap.add_argument("--serve-batch", type=int, default=48)
     ap.add_argument("--serve-streams", type=int, default=3)
     ap.add_argument("--serve-budget-gb", type=int, default=48)
@@ -193,7 +201,12 @@ def main():
                      "traffic": measured_traffic(kernel), "kernel": kernel,
                      "bytes_per_launch": round(mean_bytes),
                      "traffic_source": traffic_profile(kernel) + " (rocprofv3 FETCH_SIZE + "
-                                       "WRITE_SIZE, copy-kernel calibrated)"},
+                                       "WRITE_SIZE, copy-kernel calibrated)",
+                     # diagnostics from the same profiles: L2 hit rate (TCC_HIT /
+                     # (TCC_HIT + TCC_MISS)) and the share of all SIMD-cycles of the
+                     # launch spent issuing VALU instructions (SQ_ACTIVE_INST_VALU)
+                     "l2_hit": profile_json(traffic_profile(kernel)).get("l2_hit"),
+                     "valu_busy": profile_json(issue_profile(kernel)).get("valu_busy")},
     }
 
     if not args.no_batch:
@@ -458,7 +471,16 @@ def cpu_baseline(hs, poses, W, H, seconds):
     dt = time.perf_counter() - t0
     v = n * W * H / dt / 1e6
     log("cpu baseline: %d frames in %.1fs on %d threads -> %.2f Mrays/s" % (n, dt, threads, v))
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": round(v, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "cpu": model, "nproc": os.cpu_count(),
             "sample": "%d frames of the 100-pose flythrough at %dx%d (%.1f s: ray map + raycast "
                       "+ compose + gap-fill), OpenMP over ray lines" % (n, W, H, dt)}
 
