@@ -33,11 +33,14 @@ struct SceneDev {
 // Ray-buffer line bitmap words for render size rs.
 __host__ __device__ inline int bitmap_words(int rs) { return (rs + 31) >> 5; }
 
+// Saturating float -> int32 with NaN -> 0, branch-free: NaN is replaced
+// first, the median-of-three clamps to [-2^31, 2^31 - 128] (the largest float
+// below 2^31), where the conversion is exact truncation, and values from 2^31
+// up select INT32_MAX.
 __device__ inline int d_f2i(float f) {
-  if (f != f) return 0;
-  if (f >= 2147483648.0f) return INT32_MAX;
-  if (f < -2147483648.0f) return INT32_MIN;
-  return (int)f;
+  const float g = f != f ? 0.0f : f;
+  const int r = (int)__builtin_amdgcn_fmed3f(g, -2147483648.0f, 2147483520.0f);
+  return g >= 2147483648.0f ? INT32_MAX : r;
 }
 
 // LDS bitmap of one line, word w of this lane at base[w * stride].

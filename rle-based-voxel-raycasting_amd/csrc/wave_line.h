@@ -665,7 +665,7 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     const uint16_t* cached = rc + lane * kRunCache + rc_delta;
     const int ccap = cand ? kRunCache - rc_delta : 0;
     static_assert(kRunCache == 40, "the run cache is fetched as five 16-byte words");
-    uint4 rc0 = make_uint4(0, 0, 0, 0), rc1 = rc0, rc2 = rc0, rc3 = rc0, rc4 = rc0;
+    uint4 rc0, rc1, rc2, rc3, rc4;  // loaded and stored only for candidates
     if (cand) {
       const uint4* src = reinterpret_cast<const uint4*>(slab + rc_base);
       rc0 = src[0];
