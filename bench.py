@@ -92,8 +92,9 @@ def main():
     ap.add_argument("--serve-streams", type=int, default=3)
     ap.add_argument("--serve-budget-gb", type=int, default=48)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--waves-per-line", type=int, choices=(1, 2), default=2,
-                    help="wavefronts walking each ray line (2: DDA and drawing overlap)")
+    ap.add_argument("--waves-per-line", type=int, choices=(0, 1, 2), default=0,
+                    help="wavefronts walking each ray line (2: DDA and drawing overlap; "
+                         "0: the library's choice, two for frames of at most 16 lines per CU)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain copy kernel over MIB MiB (for counter calibration)")
     args = ap.parse_args()
@@ -113,7 +114,6 @@ def main():
 
     ctx = rlevox.Context(0)
     ctx.set_tuning(waves_per_block=args.waves_per_line)
-    kernel = "k_raycast_wave2" if args.waves_per_line == 2 else "k_raycast_wave"
     if args.calibrate_copy:
         nb = args.calibrate_copy << 20
         src, dst = ctx.alloc(nb), ctx.alloc(nb)
@@ -130,6 +130,9 @@ def main():
     stream = ctx.stream()
     rms = [rlevox.raymap(p, W, H) for p in poses]
     max_lines = max(r.map_line_count for r in rms)
+    # the kernel the library runs for these frames (MI355X: 256 CUs)
+    two = args.waves_per_line == 2 or (args.waves_per_line == 0 and max_lines <= 16 * 256)
+    kernel = "k_raycast_wave2" if two else "k_raycast_wave"
     d_rb = ctx.alloc(max_lines * W * 4)
     d_clip = ctx.alloc(max_lines * 8)
     d_c0 = ctx.alloc(W * H * 2)

@@ -172,9 +172,11 @@ typedef struct {
   int32_t waves_per_block; /* 1..4; with lines_per_wave 1: 1 = one wavefront
                               both walks and draws a line, 2..4 = two
                               wavefronts per line, one walking the DDA ahead
-                              of the one drawing (single renders: the
-                              batched kernels keep one wavefront per line,
-                              which uses a full GPU better) */
+                              of the one drawing, 0 = two when the frame
+                              has at most 16 lines per compute unit, else
+                              one (single renders: the batched kernels keep
+                              one wavefront per line, which uses a full GPU
+                              better) */
   int32_t persistent_waves; /* wavefronts per CU of the persistent batch kernel
                                (0 = default 16, else 4..32) */
   int32_t order;           /* 0 = view order, 1 = longest views first */

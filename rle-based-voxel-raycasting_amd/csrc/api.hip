@@ -169,7 +169,7 @@ rlevox_status rlevox_render_raybuf(rlevox_ctx* ctx, const rlevox_scene* scene,
     return fail(RLEVOX_ERR_INVALID_ARG, "render_raybuf: null argument");
   st = check_raymap(*rm);
   if (st != RLEVOX_OK) return st;
-  hipError_t e = launch_raycast_plain(scene->dev, *rm, d_raybuf, d_line_clip, ctx->tuning_block(),
+  hipError_t e = launch_raycast_plain(scene->dev, *rm, d_raybuf, d_line_clip, ctx->tuning_block(rm->map_line_count),
                                       (hipStream_t)stream);
   return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "raycast launch");
 }
@@ -189,7 +189,7 @@ rlevox_status rlevox_raycast_counters(rlevox_ctx* ctx, const rlevox_scene* scene
   unsigned long long* d = (unsigned long long*)ctx->counts.ptr;
   hipError_t e = launch_fill_u32((uint32_t*)d, 0, sizeof(rlevox_counters) / 4, nullptr);
   if (e == hipSuccess)
-    e = launch_raycast_plain(scene->dev, *rm, d_raybuf, d_line_clip, ctx->tuning_block(), nullptr, d);
+    e = launch_raycast_plain(scene->dev, *rm, d_raybuf, d_line_clip, ctx->tuning_block(rm->map_line_count), nullptr, d);
   if (e == hipSuccess) e = hipMemcpy(out, d, sizeof(rlevox_counters), hipMemcpyDeviceToHost);
   return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "raycast_counters");
 }
@@ -246,7 +246,7 @@ rlevox_status rlevox_render(rlevox_ctx* ctx, const rlevox_scene* scene, const rl
   hipStream_t s = (hipStream_t)stream;
   uint32_t* raybuf = (uint32_t*)ctx->raybuf.ptr;
   int32_t* clip = (int32_t*)ctx->clip.ptr;
-  hipError_t e = launch_raycast_plain(scene->dev, rm, raybuf, clip, ctx->tuning_block(), s);
+  hipError_t e = launch_raycast_plain(scene->dev, rm, raybuf, clip, ctx->tuning_block(rm.map_line_count), s);
   if (e != hipSuccess) return hip_fail(e, "raycast launch");
   uint16_t* cmp = tb ? (uint16_t*)ctx->ctmp.ptr : d_colour;
   e = launch_compose(rm, raybuf, clip, cmp, d_depth, s);

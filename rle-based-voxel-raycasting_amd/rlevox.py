@@ -416,7 +416,7 @@ class Context:
         check(lib.rlevox_event_elapsed_ms(self.handle, a, b, C.byref(ms)))
         return float(ms.value)
 
-    def set_tuning(self, lines_per_wave=1, waves_per_block=2, persistent_waves=0, order=1):
+    def set_tuning(self, lines_per_wave=1, waves_per_block=0, persistent_waves=0, order=1):
         t = Tuning(lines_per_wave, waves_per_block, persistent_waves, order)
         check(lib.rlevox_set_tuning(self.handle, C.byref(t)))
 
