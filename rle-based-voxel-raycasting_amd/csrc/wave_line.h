@@ -678,14 +678,19 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     // ---- next chunk's DDA (the run cache loads are in flight meanwhile)
     if constexpr (decltype(produce)::value) have_next = dda_next_chunk(wk, nxt, sc, zfar, dl, lane);
 
-    if (cand) {
-      uint4* dst = reinterpret_cast<uint4*>(rc + lane * kRunCache);
-      dst[0] = rc0;
-      dst[1] = rc1;
-      dst[2] = rc2;
-      dst[3] = rc3;
-      dst[4] = rc4;
-    }
+    // The stores wait for the loads: a walking wavefront stores after the DDA
+    // above, a drawing-only one after the span parameters below.
+    auto store_rc = [&]() {
+      if (cand) {
+        uint4* dst = reinterpret_cast<uint4*>(rc + lane * kRunCache);
+        dst[0] = rc0;
+        dst[1] = rc1;
+        dst[2] = rc2;
+        dst[3] = rc3;
+        dst[4] = rc4;
+      }
+    };
+    if constexpr (decltype(produce)::value) store_rc();
     // ---- next chunk's pointer gather: in flight during this chunk's replay
     if constexpr (decltype(produce)::value) {
       if (have_next) dda_gather(wk, nxt, vpx, vpz, fix_xf, fix_yf, lane);
@@ -720,6 +725,7 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
         onedz2F /= s2r - s1rF;
       }
     }
+    if constexpr (!decltype(produce)::value) store_rc();
     wave_sync();  // run cache filled
     if (cand) {  // F2: projected top of the next element the loop tests
       int yg = ygF;
