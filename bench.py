@@ -94,7 +94,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--waves-per-line", type=int, choices=(0, 1, 2), default=0,
                     help="wavefronts walking each ray line (2: DDA and drawing overlap; "
-                         "0: the library's choice, two for frames of at most 16 lines per CU)")
+                         "0: the library's choice, two for frames of at most 14 lines per CU)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain copy kernel over MIB MiB (for counter calibration)")
     args = ap.parse_args()
@@ -131,7 +131,7 @@ def main():
     rms = [rlevox.raymap(p, W, H) for p in poses]
     max_lines = max(r.map_line_count for r in rms)
     # the kernel the library runs for these frames (MI355X: 256 CUs)
-    two = args.waves_per_line == 2 or (args.waves_per_line == 0 and max_lines <= 16 * 256)
+    two = args.waves_per_line == 2 or (args.waves_per_line == 0 and max_lines <= 14 * 256)
     kernel = "k_raycast_wave2" if two else "k_raycast_wave"
     d_rb = ctx.alloc(max_lines * W * 4)
     d_clip = ctx.alloc(max_lines * 8)
@@ -298,7 +298,7 @@ def bench_batched(ctx, rlevox, workloads, scenes_h, scenes):
 
 
 def bench_c3(ctx, rlevox, workloads, frames=8, reps=3):
-    """Config 3: plain kernel on a 16384x16384x1024 map (mips 0-7, level 0
+    """Plain kernel on a 16384x16384x1024 map (mips 0-7, level 0
     beyond 2^32 slab words) at 3840x2160, grazing poses |pitch| 0.6-5 deg."""
     W, H = 3840, 2160
     t0 = time.time()

@@ -173,7 +173,7 @@ typedef struct {
                               both walks and draws a line, 2..4 = two
                               wavefronts per line, one walking the DDA ahead
                               of the one drawing, 0 = two when the frame
-                              has at most 16 lines per compute unit, else
+                              has at most 14 lines per compute unit, else
                               one (single renders: the batched kernels keep
                               one wavefront per line, which uses a full GPU
                               better) */

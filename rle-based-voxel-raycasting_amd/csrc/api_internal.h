@@ -101,12 +101,14 @@ struct rlevox_ctx {
   rlevox::BatchSlot batch;
   rlevox_tuning tuning{1, 0, 0, 1};
   // Lines per CU up to which the automatic choice walks each line with two
-  // wavefronts: a frame with few lines is bound by its longest lines, where
-  // overlapping the DDA with the drawing pays; with more lines the machine is
-  // full and one wavefront per line does the same work with fewer
-  // instructions (1080p: ~2600 lines, 8% faster with two; 4K grazing:
-  // ~10000 lines, 7% faster with one).
-  static constexpr int kTwoWaveLinesPerCU = 16;
+  // wavefronts: the two-wavefront kernel keeps 3.5 lines per SIMD resident
+  // (registers for 6 wavefronts, LDS for 14 blocks per CU), the
+  // one-wavefront kernel 4.  A frame whose lines all fit at once is bound by
+  // its longest lines, where overlapping the DDA with the drawing pays
+  // (1080p flythrough, 2000-3300 lines: 10% faster with two); beyond that a
+  // second round of long lines costs more than the overlap gains (4K grazing
+  // views, ~4100 long lines: 7% faster with one).
+  static constexpr int kTwoWaveLinesPerCU = 14;
   // 1 selects the wavefront-per-line kernel, 2 its two-wavefront form;
   // otherwise lanes per block (16 or more).
   int tuning_block(int lines) const {
