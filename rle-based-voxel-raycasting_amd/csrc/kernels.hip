@@ -24,9 +24,11 @@ __global__ __launch_bounds__(256) void k_raycast_plain(SceneDev sc, rlevox_rayma
   render_line<COUNT>(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs, clip + 2 * x, bits, counts);
 }
 
-// One wavefront (= one block) per ray line.
+// One wavefront (= one block) per ray line.  Registers are capped for four
+// wavefronts per SIMD (4096 resident lines on 256 CUs); uncapped the
+// compiler takes 140 and only three fit.
 template <bool COUNT>
-__global__ __launch_bounds__(64) void k_raycast_wave(SceneDev sc, rlevox_raymap rm,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_raycast_wave(SceneDev sc, rlevox_raymap rm,
                                                      uint32_t* __restrict__ raybuf,
                                                      int32_t* __restrict__ clip,
                                                      unsigned long long* counts) {
