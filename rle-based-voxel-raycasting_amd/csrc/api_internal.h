@@ -101,13 +101,12 @@ struct rlevox_ctx {
   rlevox::BatchSlot batch;
   rlevox_tuning tuning{1, 0, 0, 1};
   // Lines per CU up to which the automatic choice walks each line with two
-  // wavefronts: the two-wavefront kernel keeps 3.5 lines per SIMD resident
-  // (registers for 6 wavefronts, LDS for 14 blocks per CU), the
-  // one-wavefront kernel 4.  A frame whose lines all fit at once is bound by
+  // wavefronts: the two-wavefront kernel keeps 12 lines per CU resident, the
+  // one-wavefront kernel 16.  A frame whose lines all fit at once is bound by
   // its longest lines, where overlapping the DDA with the drawing pays
-  // (1080p flythrough, 2000-3300 lines: 10% faster with two); beyond that a
-  // second round of long lines costs more than the overlap gains (4K grazing
-  // views, ~4100 long lines: 7% faster with one).
+  // (1080p flythrough, 2000-3300 lines: 13% faster with two); beyond that a
+  // second round of long lines costs more than the overlap gains (4K
+  // flythrough, up to 6600 lines: 8% faster with one).
   static constexpr int kTwoWaveLinesPerCU = 14;
   // 1 selects the wavefront-per-line kernel, 2 its two-wavefront form;
   // otherwise lanes per block (16 or more).
