@@ -193,3 +193,33 @@ def test_offset_shift_scene(ctx, oracle, c1_scene, shift):
             check_view(ctx, oracle, dev, c1_scene.mips, p, 512, 384)
     finally:
         dev.free()
+
+
+@pytest.mark.parametrize("W,H", [(333, 97), (64, 16), (1000, 17), (1920, 1080)])
+def test_gapfill_arbitrary_depths(ctx, oracle, W, H):
+    """Gap-fill on images no render produces: depths around every decision
+    boundary of the filter (radius cut-off near 1874, tap limit 10*dmin/7),
+    negative and zero depths, depths above 16 bits and the sky value, on
+    sizes that do not divide into whole tiles."""
+    rng = np.random.default_rng(W * 7 + H)
+    near = rng.integers(1, 2800, size=(H, W))
+    edge = rng.integers(1860, 1890, size=(H, W))
+    big = rng.integers(60000, 1 << 20, size=(H, W))
+    neg = rng.integers(-5, 2, size=(H, W))
+    sky = np.full((H, W), 0x7FFFFFFF)
+    pick = rng.integers(0, 10, size=(H, W))
+    dep = np.where(pick < 5, near, np.where(pick < 7, edge, np.where(pick < 8, big,
+                   np.where(pick < 9, neg, sky)))).astype(np.int32)
+    col = rng.integers(0, 1 << 16, size=(H, W)).astype(np.uint16)
+    want = oracle.gapfill(col, dep)
+    d_col, d_dep, d_out = ctx.alloc(W * H * 2), ctx.alloc(W * H * 4), ctx.alloc(W * H * 2)
+    try:
+        d_col.upload(col)
+        d_dep.upload(dep)
+        ctx.gapfill(W, H, d_col, d_dep, d_out)
+        ctx.sync()
+        got = d_out.download(np.uint16, W * H).reshape(H, W)
+    finally:
+        for b in (d_col, d_dep, d_out):
+            b.free()
+    assert_same("gap-fill %dx%d" % (W, H), got, want)
