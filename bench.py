@@ -350,7 +350,7 @@ C5_SIZES = [((640, 480), 0.5), ((1920, 1080), 0.35), ((3840, 2160), 0.15)]
 
 
 def bench_served(ctx, rlevox, workloads, scenes_h, scenes, args):
-    """Config 5: 12 resident scenes, Poisson arrivals, mixed resolutions.
+    """Served requests: 12 resident scenes, Poisson arrivals, mixed resolutions.
 
     The 12 resident scenes are the 4 generated c2 scenes uploaded 3 times
     each (distinct device copies).  Requests draw a scene, a seeded pose and a

@@ -1,10 +1,11 @@
-"""Synthetic workloads: seeded scenes and camera poses of the benchmark configs.
+"""Synthetic workloads: seeded scenes and camera poses.  Scene presets:
 
-C1  512x512 columns x 256, mips 0-3, one 512x512 view (CPU-sized case).
-C2  4096x4096 x 1024, mips 0-5, 100-frame flythrough at 1920x1080.
-C3  16384x16384 x 1024, mips 0-7, grazing 3840x2160 views.
-C4  64 views over 4 C2-type scenes, resolutions uniform over 4 sizes.
-C5  request stream over resident scenes, resolutions weighted 0.5/0.35/0.15.
+c1  512x512 columns x 256, mips 0-3, one 512x512 view (CPU-sized case).
+c2  4096x4096 x 1024, mips 0-5, 100-frame flythrough at 1920x1080.
+c3  16384x16384 x 1024, mips 0-7, grazing 3840x2160 views.
+View mixes: 64 views over 4 c2 scenes with resolutions uniform over 4 sizes
+(c4_views), and request streams over resident scenes with resolutions
+weighted 0.5/0.35/0.15.
 
 A camera at height `alt` above the terrain surface at column (x, z) has
 pos[1] = alt - surface_row(x, z): rows grow downward inside a column and the

@@ -1,7 +1,7 @@
-"""Config 3 at full size (GPU): a 16384x16384x1024 seeded terrain with mips
-0-7, whose level 0 needs more than 2^32 slab words (so the encoder widens
-its offsets), rendered at 3840x2160 along grazing poses (|pitch| 0.6-5
-degrees) and compared bit for bit with the oracle."""
+"""The 16384x16384x1024 terrain at full size (GPU): a seeded terrain with
+mips 0-7, whose level 0 needs more than 2^32 slab words (so the encoder
+widens its offsets), rendered at 3840x2160 along grazing poses (|pitch|
+0.6-5 degrees) and compared bit for bit with the oracle."""
 import os
 
 import numpy as np
@@ -56,5 +56,15 @@ def test_c3_grazing_4k_parity(ctx, oracle, c3):
             assert_same("depth", got["depth"], ref_dep)
             assert_same("gap-filled colour", got["colour"], ref_fin)
             assert np.mean(ref_dep < 0x7FFFFFFF) > 0.05  # the views see terrain
+            # both kernel shapes, forced, give the same frame (the automatic
+            # choice above takes one wavefront per line at this line count)
+            for wpl in (1, 2):
+                ctx.set_tuning(waves_per_block=wpl)
+                try:
+                    other = gpu_stages(ctx, dev, rm)
+                finally:
+                    ctx.set_tuning()
+                for k in ("clip", "raybuf", "compose", "depth", "colour"):
+                    assert_same("%s, %d wavefront(s) per line" % (k, wpl), other[k], got[k])
     finally:
         dev.free()
