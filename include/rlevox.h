@@ -299,11 +299,21 @@ rlevox_status rlevox_render_batch_graph(rlevox_ctx* ctx, const rlevox_view* view
                                         int32_t nviews, const rlevox_render_params* params,
                                         void* stream);
 
+/* The context's tuning: used by batches and the server, and by single
+ * renders of a (scene, W, H) the autotuner has not measured.  Setting it
+ * also forgets every autotuned (scene, W, H) entry. */
 rlevox_status rlevox_set_tuning(rlevox_ctx* ctx, const rlevox_tuning* t);
 rlevox_status rlevox_get_tuning(rlevox_ctx* ctx, rlevox_tuning* t);
+/* The tuning a single render of (scene, W, H) uses (no counterpart in the
+ * reference, whose launch shape is fixed at compile time). */
+rlevox_status rlevox_get_tuning_for(rlevox_ctx* ctx, const rlevox_scene* scene, int32_t W,
+                                    int32_t H, rlevox_tuning* t);
 
-/* Autotuner: times the candidate tunings on `nviews` sample views and keeps
- * the fastest for (scene, W, H); returns it in *best. */
+/* Autotuner: for each (scene, W, H) among the `nviews` sample views, times
+ * the candidate tunings on those views and keeps the fastest for later
+ * renders of that (scene, W, H) (dropped when the scene is freed or the
+ * tuning is set).  *best receives the first view's winner, which also
+ * becomes the context's tuning. */
 rlevox_status rlevox_autotune(rlevox_ctx* ctx, const rlevox_view* views, int32_t nviews,
                               const rlevox_render_params* params, rlevox_tuning* best,
                               double* best_ms);

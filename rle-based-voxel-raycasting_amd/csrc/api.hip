@@ -151,6 +151,11 @@ rlevox_status rlevox_scene_upload(rlevox_ctx* ctx, const rlevox_mip* mips, int32
 
 void rlevox_scene_free(rlevox_scene* s) {
   if (!s) return;
+  if (s->ctx) {
+    std::lock_guard<std::mutex> lock(s->ctx->mu);
+    for (auto it = s->ctx->tuned.begin(); it != s->ctx->tuned.end();)
+      it = it->first.scene == s ? s->ctx->tuned.erase(it) : std::next(it);
+  }
   if (s->ctx) hipSetDevice(s->ctx->device);
   hipDeviceSynchronize();
   for (void* p : s->allocs) hipFree(p);
@@ -169,8 +174,13 @@ rlevox_status rlevox_render_raybuf(rlevox_ctx* ctx, const rlevox_scene* scene,
     return fail(RLEVOX_ERR_INVALID_ARG, "render_raybuf: null argument");
   st = check_raymap(*rm);
   if (st != RLEVOX_OK) return st;
-  hipError_t e = launch_raycast_plain(scene->dev, *rm, d_raybuf, d_line_clip, ctx->tuning_block(rm->map_line_count),
-                                      (hipStream_t)stream);
+  rlevox_tuning t;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    t = ctx->tuning_for(scene, rm->W, rm->H);
+  }
+  hipError_t e = launch_raycast_plain(scene->dev, *rm, d_raybuf, d_line_clip,
+                                      ctx->tuning_block(t, rm->map_line_count), (hipStream_t)stream);
   return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "raycast launch");
 }
 
@@ -189,7 +199,9 @@ rlevox_status rlevox_raycast_counters(rlevox_ctx* ctx, const rlevox_scene* scene
   unsigned long long* d = (unsigned long long*)ctx->counts.ptr;
   hipError_t e = launch_fill_u32((uint32_t*)d, 0, sizeof(rlevox_counters) / 4, nullptr);
   if (e == hipSuccess)
-    e = launch_raycast_plain(scene->dev, *rm, d_raybuf, d_line_clip, ctx->tuning_block(rm->map_line_count), nullptr, d);
+    e = launch_raycast_plain(scene->dev, *rm, d_raybuf, d_line_clip,
+                             ctx->tuning_block(ctx->tuning_for(scene, rm->W, rm->H), rm->map_line_count),
+                             nullptr, d);
   if (e == hipSuccess) e = hipMemcpy(out, d, sizeof(rlevox_counters), hipMemcpyDeviceToHost);
   return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "raycast_counters");
 }
@@ -246,7 +258,8 @@ rlevox_status rlevox_render(rlevox_ctx* ctx, const rlevox_scene* scene, const rl
   hipStream_t s = (hipStream_t)stream;
   uint32_t* raybuf = (uint32_t*)ctx->raybuf.ptr;
   int32_t* clip = (int32_t*)ctx->clip.ptr;
-  hipError_t e = launch_raycast_plain(scene->dev, rm, raybuf, clip, ctx->tuning_block(rm.map_line_count), s);
+  hipError_t e = launch_raycast_plain(scene->dev, rm, raybuf, clip,
+                                      ctx->tuning_block(ctx->tuning_for(scene, W, H), rm.map_line_count), s);
   if (e != hipSuccess) return hip_fail(e, "raycast launch");
   uint16_t* cmp = tb ? (uint16_t*)ctx->ctmp.ptr : d_colour;
   e = launch_compose(rm, raybuf, clip, cmp, d_depth, s);
@@ -266,13 +279,24 @@ rlevox_status rlevox_set_tuning(rlevox_ctx* ctx, const rlevox_tuning* t) {
   if (!tuning_valid(*t)) return fail(RLEVOX_ERR_INVALID_ARG, "set_tuning: value out of range");
   std::lock_guard<std::mutex> lock(ctx->mu);
   ctx->tuning = *t;
+  ctx->tuned.clear();
   return RLEVOX_OK;
 }
 
 rlevox_status rlevox_get_tuning(rlevox_ctx* ctx, rlevox_tuning* t) {
   clear_error();
   if (!ctx || !t) return fail(RLEVOX_ERR_INVALID_ARG, "get_tuning: null argument");
+  std::lock_guard<std::mutex> lock(ctx->mu);
   *t = ctx->tuning;
+  return RLEVOX_OK;
+}
+
+rlevox_status rlevox_get_tuning_for(rlevox_ctx* ctx, const rlevox_scene* scene, int32_t W,
+                                    int32_t H, rlevox_tuning* t) {
+  clear_error();
+  if (!ctx || !scene || !t) return fail(RLEVOX_ERR_INVALID_ARG, "get_tuning_for: null argument");
+  std::lock_guard<std::mutex> lock(ctx->mu);
+  *t = ctx->tuning_for(scene, W, H);
   return RLEVOX_OK;
 }
 

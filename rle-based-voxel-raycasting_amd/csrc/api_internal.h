@@ -100,6 +100,22 @@ struct rlevox_ctx {
   rlevox::Scratch raybuf, clip, ctmp, counts;
   rlevox::BatchSlot batch;
   rlevox_tuning tuning{1, 0, 0, 1};
+  // Autotuned settings per (scene, W, H); others use `tuning`.
+  struct TuneKey {
+    const rlevox_scene* scene;
+    int W, H;
+    bool operator<(const TuneKey& o) const {
+      if (scene != o.scene) return scene < o.scene;
+      if (W != o.W) return W < o.W;
+      return H < o.H;
+    }
+  };
+  std::map<TuneKey, rlevox_tuning> tuned;
+  // The tuning of a single render (caller holds mu).
+  rlevox_tuning tuning_for(const rlevox_scene* s, int W, int H) const {
+    auto it = tuned.find(TuneKey{s, W, H});
+    return it == tuned.end() ? tuning : it->second;
+  }
   // Lines per CU up to which the automatic choice walks each line with two
   // wavefronts: the two-wavefront kernel keeps 12 lines per CU resident, the
   // one-wavefront kernel 16.  A frame whose lines all fit at once is bound by
@@ -110,13 +126,13 @@ struct rlevox_ctx {
   static constexpr int kTwoWaveLinesPerCU = 14;
   // 1 selects the wavefront-per-line kernel, 2 its two-wavefront form;
   // otherwise lanes per block (16 or more).
-  int tuning_block(int lines) const {
-    if (tuning.lines_per_wave == 1) {
-      if (tuning.waves_per_block == 0)
+  int tuning_block(const rlevox_tuning& t, int lines) const {
+    if (t.lines_per_wave == 1) {
+      if (t.waves_per_block == 0)
         return lines <= kTwoWaveLinesPerCU * (num_cus > 0 ? num_cus : 256) ? 2 : 1;
-      return tuning.waves_per_block >= 2 ? 2 : 1;
+      return t.waves_per_block >= 2 ? 2 : 1;
     }
-    return tuning.lines_per_wave * tuning.waves_per_block;
+    return t.lines_per_wave * t.waves_per_block;
   }
   void release_all() {
     raybuf.release();

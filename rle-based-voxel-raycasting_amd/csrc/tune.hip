@@ -55,37 +55,70 @@ extern "C" rlevox_status rlevox_autotune(rlevox_ctx* ctx, const rlevox_view* vie
     if (!views[i].scene || !views[i].d_colour || !views[i].d_depth)
       return fail(RLEVOX_ERR_INVALID_ARG, "autotune: view %d has a null pointer", i);
   rlevox_tuning saved;
-  rlevox_get_tuning(ctx, &saved);
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    saved = ctx->tuning;
+  }
+  // the views grouped by (scene, W, H), in order of first appearance
+  std::vector<rlevox_ctx::TuneKey> keys;
+  std::vector<std::vector<rlevox_view>> groups;
+  for (int i = 0; i < nviews; i++) {
+    const rlevox_ctx::TuneKey k{views[i].scene, views[i].W, views[i].H};
+    size_t g = 0;
+    while (g < keys.size() && (keys[g] < k || k < keys[g])) g++;
+    if (g == keys.size()) {
+      keys.push_back(k);
+      groups.emplace_back();
+    }
+    groups[g].push_back(views[i]);
+  }
   hipStream_t s = nullptr;
   hipEvent_t a = nullptr, b = nullptr;
   hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&a);
   if (e == hipSuccess) e = hipEventCreate(&b);
-  if (e != hipSuccess) {
-    st = hip_fail(e, "autotune setup");
-  } else {
+  if (e != hipSuccess) st = hip_fail(e, "autotune setup");
+  const int ncand = (int)(sizeof(kCandidates) / sizeof(kCandidates[0]));
+  for (size_t g = 0; st == RLEVOX_OK && g < keys.size(); g++) {
     double best_t = 0;
     int best_i = -1;
-    for (int c = 0; c < (int)(sizeof(kCandidates) / sizeof(kCandidates[0])); c++) {
+    for (int c = 0; c < ncand; c++) {
       rlevox_tuning t = kCandidates[c];
+      t.persistent_waves = saved.persistent_waves;
       t.order = saved.order;
-      if ((st = rlevox_set_tuning(ctx, &t)) != RLEVOX_OK) break;
+      {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        ctx->tuned[keys[g]] = t;
+      }
       double ms = 0;
+      const rlevox_view* gv = groups[g].data();
+      const int gn = (int)groups[g].size();
       // one untimed pass (code-object load, scratch growth), then the timed ones
-      if ((st = time_views(ctx, views, nviews, params, s, a, b, 1, &ms)) != RLEVOX_OK) break;
-      if ((st = time_views(ctx, views, nviews, params, s, a, b, 2, &ms)) != RLEVOX_OK) break;
+      if ((st = time_views(ctx, gv, gn, params, s, a, b, 1, &ms)) != RLEVOX_OK) break;
+      if ((st = time_views(ctx, gv, gn, params, s, a, b, 2, &ms)) != RLEVOX_OK) break;
       if (best_i < 0 || ms < best_t) {
         best_t = ms;
         best_i = c;
       }
     }
-    if (st == RLEVOX_OK) {
-      *best = kCandidates[best_i];
-      best->order = saved.order;
+    if (st != RLEVOX_OK) break;
+    rlevox_tuning t = kCandidates[best_i];
+    t.persistent_waves = saved.persistent_waves;
+    t.order = saved.order;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->tuned[keys[g]] = t;
+    if (g == 0) {
+      *best = t;
       if (best_ms) *best_ms = best_t;
-      st = rlevox_set_tuning(ctx, best);
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (st == RLEVOX_OK) {
+      ctx->tuning = *best;  // also the default for keys not tuned here
     } else {
-      rlevox_set_tuning(ctx, &saved);
+      for (const auto& k : keys) ctx->tuned.erase(k);
+      ctx->tuning = saved;
     }
   }
   if (a) hipEventDestroy(a);

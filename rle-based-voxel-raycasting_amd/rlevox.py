@@ -152,6 +152,7 @@ _sigs = {
     "rlevox_render_batch_graph": (C.c_int, [_vp, P(View), C.c_int32, P(RenderParams), _vp]),
     "rlevox_set_tuning": (C.c_int, [_vp, P(Tuning)]),
     "rlevox_get_tuning": (C.c_int, [_vp, P(Tuning)]),
+    "rlevox_get_tuning_for": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, P(Tuning)]),
     "rlevox_autotune": (C.c_int, [_vp, P(View), C.c_int32, P(RenderParams), P(Tuning), P(C.c_double)]),
     "rlevox_server_create": (C.c_int, [_vp, P(ServerConfig), P(_vp)]),
     "rlevox_server_submit": (C.c_int, [_vp, _vp, P(Pose), C.c_int32, C.c_int32, _vp, _vp,
@@ -423,6 +424,12 @@ class Context:
     def get_tuning(self):
         t = Tuning()
         check(lib.rlevox_get_tuning(self.handle, C.byref(t)))
+        return t
+
+    def get_tuning_for(self, scene, W, H):
+        """The tuning a single render of (scene, W, H) uses."""
+        t = Tuning()
+        check(lib.rlevox_get_tuning_for(self.handle, scene.handle, W, H, C.byref(t)))
         return t
 
     # Stage entry points on device buffers.

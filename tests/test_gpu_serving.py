@@ -181,6 +181,38 @@ def test_autotune_keeps_results(ctx, scenes, c1_scene):
         d.free()
 
 
+def test_autotune_per_scene_and_resolution(ctx, scenes, c1_scene, tiny_scene):
+    """Each (scene, W, H) among the sample views keeps its own winner; the
+    first one also becomes the context's tuning; setting a tuning forgets
+    them; renders stay identical throughout."""
+    spec = [(0, 1280, 720), (1, 640, 480), (0, 1280, 720), (1, 640, 480)]
+    hosts = [c1_scene, tiny_scene]
+    poses = [workloads.random_views(hosts[si].mips[0], 1, seed=40 + i,
+                                    alt=(2.0, 40.0) if si else (10.0, 120.0))[0]
+             for i, (si, W, H) in enumerate(spec)]
+    bufs = [(ctx.alloc(W * H * 2), ctx.alloc(W * H * 4)) for _, W, H in spec]
+    views = [rlevox.make_view(scenes[si], p, W, H, c, d)
+             for (si, W, H), p, (c, d) in zip(spec, poses, bufs)]
+    want = [plain_render(ctx, scenes[si], p, W, H) for (si, W, H), p in zip(spec, poses)]
+    try:
+        best, ms = ctx.autotune(views)
+        assert ms > 0
+        t0 = ctx.get_tuning_for(scenes[0], 1280, 720).as_tuple()
+        t1 = ctx.get_tuning_for(scenes[1], 640, 480).as_tuple()
+        assert t0 == best.as_tuple() == ctx.get_tuning().as_tuple()
+        assert t1[0] in (1, 16, 32, 64) and t1[1] in (1, 2, 4)
+        for (si, W, H), p, w in zip(spec, poses, want):
+            got = plain_render(ctx, scenes[si], p, W, H)
+            assert np.array_equal(got[0], w[0]) and np.array_equal(got[1], w[1])
+        ctx.set_tuning(lines_per_wave=64, waves_per_block=1)
+        assert ctx.get_tuning_for(scenes[1], 640, 480).as_tuple()[:2] == (64, 1)
+    finally:
+        ctx.set_tuning()
+        for c, d in bufs:
+            c.free()
+            d.free()
+
+
 def test_device_delivery(ctx, scenes, c1_scene, tiny_scene):
     reqs = requests(c1_scene, tiny_scene, 16, seed=31)
     bufs = [(ctx.alloc(W * H * 2), ctx.alloc(W * H * 4)) for _, _, W, H in reqs]
