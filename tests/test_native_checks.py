@@ -1,6 +1,6 @@
-"""Host check of the wavefront kernel's chunked DDA: lane k holding term k
-of each crossing sequence and the steps ordered by merge ranks reproduce
-the sequential walk bit for bit (CPU)."""
+"""Native host checks (CPU): the wavefront kernel's chunked DDA against the
+sequential walk, and the library's host code plus the oracle under
+AddressSanitizer and UndefinedBehaviorSanitizer."""
 import os
 import subprocess
 
@@ -8,6 +8,8 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 NATIVE = os.path.join(HERE, "native")
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(ROOT, "rle-based-voxel-raycasting_amd", "csrc")
 
 
 def build_and_run(name, arg, tmp_path):
@@ -22,3 +24,21 @@ def test_chunked_dda_matches_sequential_walk(tmp_path):
     rc, out = build_and_run("dda_chunk_check", 3000, tmp_path)
     assert rc == 0, out
     assert "mismatches 0" in out
+
+
+def test_host_code_under_sanitizers(tmp_path):
+    """Scene generation, validation (with 200 corrupted scenes), pointer-map
+    rebuild, re-packing, .rle4 files (with a corrupted one), the dense
+    encoder, 2000 random ray maps and oracle renders, built with
+    -fsanitize=address,undefined: no report, every check passes."""
+    exe = str(tmp_path / "host_sanitize")
+    srcs = [os.path.join(NATIVE, "host_sanitize.cpp")]
+    srcs += [os.path.join(CSRC, f) for f in ("host.cpp", "raymap.cpp", "scene.cpp")]
+    srcs += [os.path.join(ROOT, "oracle", f) for f in ("oracle.cpp", "brute.cpp")]
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-pthread",
+                    "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+                    "-fno-omit-frame-pointer"] + srcs + ["-o", exe], check=True)
+    r = subprocess.run([exe, str(tmp_path)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, env=dict(os.environ, OMP_NUM_THREADS="2"))
+    assert r.returncode == 0, r.stdout[-4000:]
+    assert r.stdout.strip().endswith("host sanitize ok"), r.stdout[-4000:]
