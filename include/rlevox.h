@@ -349,6 +349,12 @@ void rlevox_server_destroy(rlevox_server* srv);
 
 rlevox_status rlevox_dev_alloc(rlevox_ctx* ctx, uint64_t bytes, void** out);
 rlevox_status rlevox_dev_free(rlevox_ctx* ctx, void* p);
+
+/* Testing aid, no counterpart in the reference: the n-th device allocation
+ * the library makes from now on (scene upload, scratch growth, dev_alloc,
+ * batch and server buffers) fails as out of memory; n <= 0 disarms.
+ * Process-wide. */
+void rlevox_debug_fail_alloc(int32_t n);
 rlevox_status rlevox_dev_fill_u32(rlevox_ctx* ctx, void* p, uint32_t value, uint64_t count, void* stream);
 /* Device-to-device copy by a plain streaming kernel (16-byte aligned
  * pointers, size a multiple of 16): exact read+write traffic, used to

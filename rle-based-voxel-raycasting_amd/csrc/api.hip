@@ -2,6 +2,7 @@
 // the device plumbing used by hosts that do not link the HIP runtime.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -12,6 +13,18 @@
 using namespace rlevox;
 
 namespace rlevox {
+
+// Device allocations of the library go through here so that a test can make
+// the n-th one fail as if the device were out of memory.
+static std::atomic<int> g_fail_alloc{0};
+
+hipError_t dev_malloc(void** p, size_t bytes) {
+  int n = g_fail_alloc.load();
+  while (n > 0 && !g_fail_alloc.compare_exchange_weak(n, n - 1)) {
+  }
+  if (n == 1) return hipErrorOutOfMemory;
+  return hipMalloc(p, bytes);
+}
 
 rlevox_status hip_fail(hipError_t e, const char* what) {
   return fail(e == hipErrorOutOfMemory ? RLEVOX_ERR_OUT_OF_MEMORY : RLEVOX_ERR_DEVICE,
@@ -33,7 +46,7 @@ rlevox_status Scratch::ensure(size_t bytes) {
     size = 0;
   }
   size_t want = bytes + bytes / 4;
-  hipError_t e = hipMalloc(&ptr, want);
+  hipError_t e = dev_malloc(&ptr, want);
   if (e != hipSuccess) {
     ptr = nullptr;
     return hip_fail(e, "hipMalloc(scratch)");
@@ -116,8 +129,8 @@ rlevox_status rlevox_scene_upload(rlevox_ctx* ctx, const rlevox_mip* mips, int32
     const size_t slab_bytes = (((size_t)m[i].nslabs * 2 + 255) & ~(size_t)255) + 512;
     void* dm = nullptr;
     void* ds = nullptr;
-    hipError_t e = hipMalloc(&dm, map_bytes);
-    if (e == hipSuccess) e = hipMalloc(&ds, slab_bytes);
+    hipError_t e = dev_malloc(&dm, map_bytes);
+    if (e == hipSuccess) e = dev_malloc(&ds, slab_bytes);
     if (e == hipSuccess) e = hipMemcpy(dm, m[i].ptrmap, map_bytes, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(ds, m[i].slabs, (size_t)m[i].nslabs * 2, hipMemcpyHostToDevice);
     if (dm) s->allocs.push_back(dm);
@@ -135,7 +148,7 @@ rlevox_status rlevox_scene_upload(rlevox_ctx* ctx, const rlevox_mip* mips, int32
     s->bytes += map_bytes + slab_bytes;
   }
   void* dd = nullptr;
-  hipError_t e = hipMalloc(&dd, sizeof(SceneDev));
+  hipError_t e = dev_malloc(&dd, sizeof(SceneDev));
   if (e == hipSuccess) {
     s->allocs.push_back(dd);
     e = hipMemcpy(dd, &s->dev, sizeof(SceneDev), hipMemcpyHostToDevice);
@@ -307,9 +320,11 @@ rlevox_status rlevox_dev_alloc(rlevox_ctx* ctx, uint64_t bytes, void** out) {
   rlevox_status st = ctx_bind(ctx);
   if (st != RLEVOX_OK) return st;
   if (!out) return fail(RLEVOX_ERR_INVALID_ARG, "dev_alloc: null out");
-  hipError_t e = hipMalloc(out, bytes ? bytes : 1);
+  hipError_t e = dev_malloc(out, bytes ? bytes : 1);
   return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "hipMalloc");
 }
+
+void rlevox_debug_fail_alloc(int32_t n) { g_fail_alloc.store(n > 0 ? n : 0); }
 
 rlevox_status rlevox_dev_free(rlevox_ctx* ctx, void* p) {
   clear_error();

@@ -22,6 +22,7 @@ struct Scratch {
 };
 
 rlevox_status hip_fail(hipError_t e, const char* what);
+hipError_t dev_malloc(void** p, size_t bytes);
 
 inline bool tuning_valid(const rlevox_tuning& t) {
   return (t.lines_per_wave == 1 || t.lines_per_wave == 16 || t.lines_per_wave == 32 ||

@@ -153,6 +153,7 @@ _sigs = {
     "rlevox_set_tuning": (C.c_int, [_vp, P(Tuning)]),
     "rlevox_get_tuning": (C.c_int, [_vp, P(Tuning)]),
     "rlevox_get_tuning_for": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, P(Tuning)]),
+    "rlevox_debug_fail_alloc": (None, [C.c_int32]),
     "rlevox_autotune": (C.c_int, [_vp, P(View), C.c_int32, P(RenderParams), P(Tuning), P(C.c_double)]),
     "rlevox_server_create": (C.c_int, [_vp, P(ServerConfig), P(_vp)]),
     "rlevox_server_submit": (C.c_int, [_vp, _vp, P(Pose), C.c_int32, C.c_int32, _vp, _vp,
