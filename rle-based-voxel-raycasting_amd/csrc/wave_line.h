@@ -615,13 +615,17 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     const uint32_t slen = ent.y & 0xffffu;
     const uint16_t* runs = slab + 2 + wofs;
 
-    // F: projected top of the first drawn element in front of the eye.
+    // F: projected top of the first drawn element in front of the eye.  A
+    // step whose cell top is already below the horizon with pz > 0 stays
+    // culled as the horizon rises (the replay's clip-top test, monotone in
+    // ymin for pz > 0): it can never act, and its F is not needed.
+    const bool culled0 = pz > 0.0f && top <= pz * (float)ymin;
     int F = INT32_MIN, kF = -1, ygF = 0;
     uint32_t nel = 0;
     float texF = 0;
     int textureF = 0;
     float ytopF = 0, ybotF = 0;
-    {
+    if (!culled0) {
       uint32_t w = ent.y >> 16;
       int yg = 0;
       float tex = 0;
@@ -659,7 +663,7 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     // stored to LDS after the next chunk's DDA (its latency overlaps that
     // work).  The horizon only rises, so F > ymin now is necessary for
     // acting; the clip-top test is monotone in ymin only for pz > 0.
-    const bool cand = valid && F > ymin && !(pz > 0.0f && top <= pz * (float)ymin);
+    const bool cand = valid && F > ymin && !culled0;
     const uint64_t rc_base = wofs & ~(uint64_t)7;  // 16-byte aligned start
     const int rc_delta = (int)(wofs - rc_base);
     const uint16_t* cached = rc + lane * kRunCache + rc_delta;
