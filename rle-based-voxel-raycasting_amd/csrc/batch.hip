@@ -55,7 +55,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     if ((int64_t)g >= total) break;
     const int v = find_view<false>(views, nv, (int64_t)g);
     const ViewDesc& vd = views[v];
-    const int x = (int)((int64_t)g - vd.line_begin);
+    // lines of a view from the last to the first, as the single-view kernels
+    const int x = vd.rm.map_line_count - 1 - (int)((int64_t)g - vd.line_begin);
     render_line_wave<false>(x, *vd.scene, vd.rm, vd.raybuf + (size_t)x * (size_t)vd.rm.rs,
                             vd.clip + 2 * x, bits, dda, reinterpret_cast<uint16_t*>(rcache),
                             sweep, nullptr);

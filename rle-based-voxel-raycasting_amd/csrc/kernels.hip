@@ -24,6 +24,12 @@ __global__ __launch_bounds__(256) void k_raycast_plain(SceneDev sc, rlevox_rayma
   render_line<COUNT>(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs, clip + 2 * x, bits, counts);
 }
 
+// Blocks take the ray lines from the last to the first: the lines of the
+// last segments of the ray map start first, which for views looking down
+// puts the long lines (those reaching the sky, walked to the far distance)
+// at the head of the launch.  Measured 1-6% faster than index order on
+// 1080p and 4K flythroughs and random views, equal on grazing views.
+//
 // One wavefront (= one block) per ray line.  Registers are capped for four
 // wavefronts per SIMD (4096 resident lines on 256 CUs); uncapped the
 // compiler takes 140 and only three fit.
@@ -36,8 +42,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   __shared__ int sweep[64];
   float* dda = reinterpret_cast<float*>(rcache);  // see kDdaLds
   extern __shared__ uint32_t lds[];
-  const int x = blockIdx.x;
-  if (x >= rm.map_line_count) return;
+  if ((int)blockIdx.x >= rm.map_line_count) return;
+  const int x = rm.map_line_count - 1 - (int)blockIdx.x;  // last line first
   render_line_wave<COUNT>(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs, clip + 2 * x, lds, dda,
                           reinterpret_cast<uint16_t*>(rcache), sweep, counts);
 }
@@ -52,8 +58,8 @@ __global__ __launch_bounds__(128) void k_raycast_wave2(SceneDev sc, rlevox_rayma
   __shared__ float dda[kDdaLds];
   __shared__ ChunkRing ring;
   extern __shared__ uint32_t lds[];
-  const int x = blockIdx.x;
-  if (x >= rm.map_line_count) return;
+  if ((int)blockIdx.x >= rm.map_line_count) return;
+  const int x = rm.map_line_count - 1 - (int)blockIdx.x;  // last line first
   render_line_wave<false, true>(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs, clip + 2 * x, lds,
                                 dda, reinterpret_cast<uint16_t*>(rcache), sweep, nullptr, &ring);
 }
