@@ -516,13 +516,31 @@ struct GapTile {
 __device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ cin,
                                     const int32_t* __restrict__ dep, uint16_t* __restrict__ cout,
                                     int x0, int y0, GapTile& t) {
-  for (int i = threadIdx.x; i < kGapTW * kGapTH; i += 256) {
-    const int ty = i / kGapTW, tx = i - ty * kGapTW;
-    const int gx = min(max(x0 - kGapHalo + tx, 0), W - 1);
-    const int gy = min(max(y0 - kGapHalo + ty, 0), H - 1);
-    const size_t g = (size_t)gy * W + gx;
-    t.d[ty][tx] = dep[g];
-    t.c[ty][tx] = cin[g];
+  // staging: every load of the thread is issued before the first store, so
+  // the loads' latencies overlap
+  constexpr int kStage = (kGapTW * kGapTH + 255) / 256;
+  int32_t dv[kStage];
+  uint16_t cv[kStage];
+#pragma unroll
+  for (int k = 0; k < kStage; k++) {
+    const int i = threadIdx.x + k * 256;
+    if (i < kGapTW * kGapTH) {
+      const int ty = i / kGapTW, tx = i - ty * kGapTW;
+      const int gx = min(max(x0 - kGapHalo + tx, 0), W - 1);
+      const int gy = min(max(y0 - kGapHalo + ty, 0), H - 1);
+      const size_t g = (size_t)gy * W + gx;
+      dv[k] = dep[g];
+      cv[k] = cin[g];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kStage; k++) {
+    const int i = threadIdx.x + k * 256;
+    if (i < kGapTW * kGapTH) {
+      const int ty = i / kGapTW, tx = i - ty * kGapTW;
+      t.d[ty][tx] = dv[k];
+      t.c[ty][tx] = cv[k];
+    }
   }
   __syncthreads();
   const int lx = (threadIdx.x & 63) + kGapHalo, ly = (threadIdx.x >> 6) + kGapHalo;
