@@ -63,18 +63,37 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   }
 }
 
+// The descriptor of the view a block is working on, copied to LDS when the
+// block's tiles move to another view (its tiles only grow, so views change
+// rarely): compose's per-tile reads of the ray map's fields then come from
+// LDS (C4 batch compose 1.50 -> 1.20 ms).
+__device__ inline void stage_view(const ViewDesc* __restrict__ views, int v, ViewDesc& vs) {
+  static_assert(sizeof(ViewDesc) % 4 == 0 && sizeof(ViewDesc) / 4 <= 256, "one word per thread");
+  __syncthreads();  // every thread is done with the previous view
+  if (threadIdx.x < sizeof(ViewDesc) / 4)
+    reinterpret_cast<uint32_t*>(&vs)[threadIdx.x] =
+        reinterpret_cast<const uint32_t*>(views + v)[threadIdx.x];
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void k_compose_batch(const BatchHeader* __restrict__ h,
                                                        const ViewDesc* __restrict__ views) {
+  __shared__ ViewDesc vd;
   const int64_t total = h->total_tiles;
   const int nv = h->nviews;
+  int v = find_view<true>(views, nv, (int64_t)blockIdx.x), staged = -1;
   for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
-    const int v = find_view<true>(views, nv, tile);
-    const ViewDesc& vd = views[v];
+    while (v + 1 < nv && views[v + 1].tile_begin <= tile) v++;  // tiles only grow
+    if (v != staged) {
+      stage_view(views, v, vd);
+      staged = v;
+    }
     const int W = vd.rm.W, H = vd.rm.H;
-    const int64_t local = tile - vd.tile_begin;
+    const int local = (int)(tile - vd.tile_begin);  // < 2^18 tiles per view
     const int tiles_x = (W + 63) / 64;
-    const int px = (int)(local % tiles_x) * 64 + (threadIdx.x & 63);
-    const int row = (int)(local / tiles_x) * 4 + (threadIdx.x >> 6);
+    const int ty = local / tiles_x;
+    const int px = (local - ty * tiles_x) * 64 + (threadIdx.x & 63);
+    const int row = ty * 4 + (threadIdx.x >> 6);
     if (px < W && row < H) {
       uint16_t c;
       int32_t d;
@@ -91,15 +110,17 @@ __global__ __launch_bounds__(256) void k_gapfill_batch(const BatchHeader* __rest
   __shared__ GapTile gt;
   const int64_t total = h->total_tiles;
   const int nv = h->nviews;
+  int v = find_view<true>(views, nv, (int64_t)blockIdx.x);
   for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
-    const int v = find_view<true>(views, nv, tile);
-    const ViewDesc& vd = views[v];
+    while (v + 1 < nv && views[v + 1].tile_begin <= tile) v++;  // tiles only grow
+    const ViewDesc& vd = views[v];  // a few fields per tile: read in place
     if (!(vd.post & RLEVOX_POST_GAPFILL)) continue;  // uniform over the block
     const int W = vd.rm.W;
-    const int64_t local = tile - vd.tile_begin;
+    const int local = (int)(tile - vd.tile_begin);  // < 2^18 tiles per view
     const int tiles_x = (W + 63) / 64;
-    gapfill_tile(W, vd.rm.H, vd.ctmp, vd.depth, vd.colour, (int)(local % tiles_x) * 64,
-                 (int)(local / tiles_x) * 4, gt);
+    const int ty = local / tiles_x;
+    gapfill_tile(W, vd.rm.H, vd.ctmp, vd.depth, vd.colour, (local - ty * tiles_x) * 64, ty * 4,
+                 gt);
   }
 }
 
