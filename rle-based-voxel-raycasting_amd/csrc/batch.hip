@@ -48,12 +48,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const int lane = threadIdx.x;
   const int64_t total = h->total_lines;
   const int nv = h->nviews;
+  int v = 0;  // a wavefront's lines only grow, so its view index only advances
   for (;;) {
     uint32_t g = 0;
     if (lane == 0) g = atomicAdd(&h->queue, 1u);
     g = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)g, 0));
     if ((int64_t)g >= total) break;
-    const int v = find_view<false>(views, nv, (int64_t)g);
+    while (v + 1 < nv && views[v + 1].line_begin <= (int64_t)g) v++;
     const ViewDesc& vd = views[v];
     // lines of a view from the last to the first, as the single-view kernels
     const int x = vd.rm.map_line_count - 1 - (int)((int64_t)g - vd.line_begin);
