@@ -91,6 +91,9 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=48)
     ap.add_argument("--serve-streams", type=int, default=3)
     ap.add_argument("--serve-budget-gb", type=int, default=48)
+    ap.add_argument("--serve-c3", type=int, default=0, metavar="K",
+                    help="make K of the 12 resident scenes device copies of the "
+                         "16384^2 x 1024 terrain (~31 GB each) instead of 4096^2 ones")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--waves-per-line", type=int, choices=(0, 1, 2), default=0,
                     help="wavefronts walking each ray line (2: DDA and drawing overlap; "
@@ -353,7 +356,8 @@ def bench_served(ctx, rlevox, workloads, scenes_h, scenes, args):
     """Served requests: 12 resident scenes, Poisson arrivals, mixed resolutions.
 
     The 12 resident scenes are the 4 generated c2 scenes uploaded 3 times
-    each (distinct device copies).  Requests draw a scene, a seeded pose and a
+    each (distinct device copies); with --serve-c3 K, K of the copies are
+    replaced by device copies of the 16384^2 x 1024 terrain.  Requests draw a scene, a seeded pose and a
     resolution (640x480 / 1920x1080 / 3840x2160 at 0.5 / 0.35 / 0.15) and are
     rendered into device framebuffers.  Three runs over the same requests:
     plain (one synchronous rlevox_render per request, back to back), served
@@ -362,6 +366,15 @@ def bench_served(ctx, rlevox, workloads, scenes_h, scenes, args):
     """
     import threading
     resident = list(scenes) + [ctx.upload(h.mips) for h in scenes_h for _ in range(2)]
+    resident_h = list(scenes_h) + [h for h in scenes_h for _ in range(2)]
+    big_h = None
+    if args.serve_c3 > 0:
+        k = min(args.serve_c3, len(resident) - len(scenes))
+        for s in resident[len(resident) - k:]:
+            s.free()
+        big_h = workloads.gen("c3", seed=1)
+        resident = resident[:len(resident) - k] + [ctx.upload(big_h.mips) for _ in range(k)]
+        resident_h = resident_h[:len(resident_h) - k] + [big_h] * k
     rng = np.random.default_rng(55)
     n = args.serve_requests
     reqs = []
@@ -369,8 +382,7 @@ def bench_served(ctx, rlevox, workloads, scenes_h, scenes, args):
         u = rng.random()
         (W, H) = C5_SIZES[0][0] if u < 0.5 else (C5_SIZES[1][0] if u < 0.85 else C5_SIZES[2][0])
         si = int(rng.integers(len(resident)))
-        pose = workloads.random_views(scenes_h[si % len(scenes_h)].mips[0], 1,
-                                      int(rng.integers(1 << 30)))[0]
+        pose = workloads.random_views(resident_h[si].mips[0], 1, int(rng.integers(1 << 30)))[0]
         reqs.append((si, pose, W, H))
     offs, total = [], 0
     for _, _, W, H in reqs:
@@ -397,6 +409,8 @@ def bench_served(ctx, rlevox, workloads, scenes_h, scenes, args):
         lat_plain.append((time.perf_counter() - t1) * 1e3)
     plain_s = time.perf_counter() - t0
     out = {"requests": n, "resident_scenes": len(resident), "rays": rays,
+           "resident_gb": round(sum(s.device_bytes for s in resident) / 1e9, 2),
+           "resident_16384_copies": args.serve_c3,
            "mix": "640x480:0.5 1920x1080:0.35 3840x2160:0.15",
            "plain_mrays_s": round(rays / plain_s / 1e6, 2)}
 
@@ -452,6 +466,8 @@ def bench_served(ctx, rlevox, workloads, scenes_h, scenes, args):
     arena.free()
     for s in resident[len(scenes):]:
         s.free()
+    if big_h is not None:
+        big_h.close()
     return out
 
 
