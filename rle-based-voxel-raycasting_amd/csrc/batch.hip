@@ -58,7 +58,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const ViewDesc& vd = views[v];
     // lines of a view from the last to the first, as the single-view kernels
     const int x = vd.rm.map_line_count - 1 - (int)((int64_t)g - vd.line_begin);
-    render_line_wave<false>(x, *vd.scene, vd.rm, vd.raybuf + (size_t)x * (size_t)vd.rm.rs,
+    render_line_wave<false, false, false>(x, *vd.scene, vd.rm,
+                                          vd.raybuf + (size_t)x * (size_t)vd.rm.rs,
                             vd.clip + 2 * x, bits, dda, reinterpret_cast<uint16_t*>(rcache),
                             sweep, nullptr);
   }

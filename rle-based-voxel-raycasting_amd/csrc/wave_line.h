@@ -106,6 +106,16 @@ __device__ inline int prefix_max_excl(int v, int init, int lane) {
   return lane == 0 ? init : max(init, prev);
 }
 
+// Inclusive prefix sum of v over lanes 0..lane.
+__device__ inline int prefix_sum_incl(int v, int lane) {
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int o = __shfl_up(v, off);
+    if (lane >= off) v += o;
+  }
+  return v;
+}
+
 // ------------------------------------------------------------ span pixels
 
 // Sets the bitmap bits of rows base..base+63 given by mask m.  Lane 0 issues
@@ -435,8 +445,11 @@ __device__ inline void ring_get(const ChunkRing& r, int s, DdaChunk& c, int lane
 // TWO: the line is walked by a 128-thread block, wavefront 0 producing the
 // DDA chunks (and their pointer gathers) two chunks ahead of wavefront 1,
 // which consumes them; the two halves of the work overlap.  Otherwise one
-// wavefront does both, software-pipelined.
-template <bool COUNT, bool TWO = false>
+// wavefront does both, software-pipelined.  PAR_ELEMS: the ordered replay
+// precomputes a column's remaining elements across the lanes (pays where
+// steps often show several elements, as in grazing views of caved terrain;
+// the persistent batch kernel leaves it off, where its registers spill).
+template <bool COUNT, bool TWO = false, bool PAR_ELEMS = true>
 __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_raymap& rm,
                                         uint32_t* __restrict__ row, int32_t* __restrict__ clip,
                                         uint32_t* bits, float* dl, uint16_t* rc, int* sweep_lds,
@@ -922,6 +935,99 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
         tex = 0;
       }
 
+      // ---- remaining elements (Cuda_Render.h:511-734)
+      if constexpr (!COUNT && PAR_ELEMS) {
+        // Lane j precomputes element k0 + base + j of the column: its run
+        // word, its rows and texture range from prefix sums over the lanes
+        // (integers, so equal to the sequential sums; tex holds integers
+        // below 2^24, so its float sums are exact too), and every projection
+        // and span parameter the sequential loop would compute for it, with
+        // the same operations in the same order.  The uniform loop below
+        // then applies the horizon clip, the bitmap and the drawing to one
+        // element after another, reading those values by lane.
+        bool stop = false;
+        for (uint32_t base = k0; base < tslen && !stop; base += 64) {
+          const uint32_t k = base + (uint32_t)lane;
+          const bool in = k < tslen;
+          uint32_t w = 0;
+          if (in) w = k == 0 ? tlf >> 16 : ((int)k + 2 < tcap ? tcache[k + 2] : truns[k]);
+          const int edrawn = (int)(w >> 10);
+          const int eskip = (int)(w & 1023u) << mip;
+          const int yinc = prefix_sum_incl(eskip + (edrawn << mip), lane);
+          const int tinc = prefix_sum_incl(edrawn, lane);
+          const int ytop_i = ygen + yinc - (edrawn << mip);  // after this element's skip
+          const int texture = (int)(tex + (float)(tinc - edrawn));
+          const float etex = tex + (float)tinc;  // tex after this element
+          const float ytop = (float)ytop_i;
+          const float ybot = (float)(ytop_i + (edrawn << mip));
+          float zz1 = tpz, yy1 = tpy;
+          if (mountain + ytop >= 0) {
+            zz1 += tczz;
+            yy1 += tcyy;
+          }
+          const float pz1 = zz1 + z_add * ytop;
+          const float py1 = yy1 + y_add * ytop;
+          const int es2 = d_f2i(half + py1 / pz1);
+          float zz2 = tpz, yy2 = tpy;
+          if (mountain + ybot < 0) {
+            zz2 += tczz;
+            yy2 += tcyy;
+          }
+          const float pz2 = zz2 + z_add * ybot;
+          const float py2 = yy2 + y_add * ybot;
+          const int es1 = d_f2i(half + py2 / pz2 - 1.0f);
+          const float pz1r = tpz + z_add * ytop;
+          const float py1r = tpy + y_add * ytop;
+          const float pz2r = tpz + z_add * ybot;
+          const float py2r = tpy + y_add * ybot;
+          const float s2r = half + py1r / pz1r;
+          const float es1r = half + py2r / pz2r;
+          const float eu1z = etex / pz2r;
+          float eu2dz = (float)texture / pz1r - eu1z;
+          const float eonez1 = 1.0f / pz2r;
+          float eonedz2 = 1.0f / pz1r - eonez1;
+          eu2dz /= s2r - es1r;
+          eonedz2 /= s2r - es1r;
+          // flags: drawn, in front at the top, in front at the bottom
+          const int eflags = (in && edrawn != 0 ? 1 : 0) | (pz1 > 0 ? 2 : 0) | (pz2 > 0 ? 4 : 0);
+          const int nb = (int)min(64u, tslen - base);
+          for (int j = 0; j < nb; j++) {
+            const int f = readlane_i(eflags, j);
+            if (!(f & 1) || !(f & 2)) continue;
+            int s2 = readlane_i(es2, j);
+            if (s2 <= ymin) {
+              stop = true;
+              break;
+            }
+            if (!(f & 4)) continue;
+            int s1 = readlane_i(es1, j);
+            if (s1 >= ymax) continue;
+            if (s2 >= ymax) {
+              s2 = ymax;
+              ymax = s1;
+            }
+            if (s1 <= ymin) {
+              s1 = ymin;
+              ymin = s2;
+              if (ymax > ymin) ymin = first_clear(bits, ymin, ymax, lane);
+            }
+            const SpanWindow sw = span_window(bits, s1, s2, lane);
+            if (sw.y0 >= s2) continue;
+            const float s1r = readlane_f(es1r, j);
+            const float u2dz = readlane_f(eu2dz, j), onedz2 = readlane_f(eonedz2, j);
+            const float mult = (float)(sw.y0 + 1) - s1r;
+            const float uzb = readlane_f(eu1z, j) + u2dz * mult;
+            const float ozb = readlane_f(eonez1, j) + onedz2 * mult;
+            span_draw(sw, s1, s2, uzb, ozb, u2dz, onedz2, (float)readlane_i(texture, j),
+                      readlane_f(etex, j) - 1.0f, tcache, tcap, tcol, 2 + (int)tslen, row, bits,
+                      lane);
+          }
+          ygen += readlane_i(yinc, 63);
+          tex += (float)readlane_i(tinc, 63);
+        }
+        cur_step = t + 1;
+        continue;
+      }
       // ---- remaining elements (Cuda_Render.h:511-734), uniform
       for (uint32_t k = k0; k < tslen; k++) {
         uint32_t w;
