@@ -283,7 +283,10 @@ rlevox_status rlevox_compose(rlevox_ctx* ctx, const rlevox_raymap* rm, const uin
 rlevox_status rlevox_gapfill(rlevox_ctx* ctx, int32_t W, int32_t H, const uint16_t* d_colour_in,
                              const int32_t* d_depth, uint16_t* d_colour_out, void* stream);
 
-/* Whole frame: pose -> framebuffer (device pointers), scratch owned by ctx. */
+/* Whole frame: pose -> framebuffer (device pointers), scratch owned by ctx.
+ * Returns when the framebuffer is complete (the context's scratch is reused
+ * by the next call), as the reference's per-frame render does; the stage
+ * entry points above are stream-ordered and return at once. */
 rlevox_status rlevox_render(rlevox_ctx* ctx, const rlevox_scene* scene, const rlevox_pose* pose,
                             int32_t W, int32_t H, const rlevox_render_params* params,
                             uint16_t* d_colour, int32_t* d_depth, void* stream);
