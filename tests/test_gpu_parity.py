@@ -223,3 +223,47 @@ def test_gapfill_arbitrary_depths(ctx, oracle, W, H):
         for b in (d_col, d_dep, d_out):
             b.free()
     assert_same("gap-fill %dx%d" % (W, H), got, want)
+
+
+# Poses inside a 64x64x1024 map whose columns hold 60-150 elements each
+# (150 cave layers on average): the oracle examines 20-70 elements per DDA
+# step on these views, so the ordered replay walks past its first 64
+# remaining elements of a column (the element preparation runs in groups
+# of 64 lanes, and this is where the groups' running height and texture
+# coordinate carry over).
+LAYERED_POSES = [(-350.0, -0.2, 0.7), (-350.0, -0.6, 2.0), (-600.0, 0.1, 1.3),
+                 (-800.0, -0.3, 4.0), (-500.0, 0.4, 5.5), (-900.0, 0.05, 3.1)]
+
+
+@pytest.fixture(scope="module")
+def layered_scene():
+    return rlevox.gen_scene(64, 1024, 3, 4, 9, 0.5, 600, 1000, 150.0, 400, 0)
+
+
+def test_many_elements_per_column(ctx, oracle, layered_scene):
+    m0 = layered_scene.mips[0]
+    counts = np.array([int(m0["slabs"][int(m0["ptrmap"][2 * c])]) for c in range(64 * 64)])
+    assert (counts > 64).mean() > 0.9 and counts.max() > 128, "scene lost its layers"
+    dev = ctx.upload(layered_scene.mips)
+    views = []
+    try:
+        for y, pitch, yaw in LAYERED_POSES:
+            p = workloads.pose(20.3, y, 30.7, pitch, yaw)
+            check_view(ctx, oracle, dev, layered_scene.mips, p, 320, 240)
+        # the persistent batch kernel on the same views
+        for y, pitch, yaw in LAYERED_POSES:
+            p = workloads.pose(20.3, y, 30.7, pitch, yaw)
+            dc, dd = ctx.alloc(320 * 240 * 2), ctx.alloc(320 * 240 * 4)
+            views.append((p, dc, dd, rlevox.make_view(dev, p, 320, 240, dc, dd)))
+        ctx.render_batch([v[3] for v in views])
+        for k, (p, dc, dd, _) in enumerate(views):
+            want = oracle.render(layered_scene.mips, p, 320, 240)
+            assert_same("batch view %d depth" % k, dd.download(np.int32, 320 * 240).reshape(240, 320),
+                        want["depth"])
+            assert_same("batch view %d colour" % k,
+                        dc.download(np.uint16, 320 * 240).reshape(240, 320), want["colour"])
+    finally:
+        for _, dc, dd, _ in views:
+            dc.free()
+            dd.free()
+        dev.free()
