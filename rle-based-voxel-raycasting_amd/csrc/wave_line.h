@@ -991,7 +991,16 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
           // flags: drawn, in front at the top, in front at the bottom
           const int eflags = (in && edrawn != 0 ? 1 : 0) | (pz1 > 0 ? 2 : 0) | (pz2 > 0 ? 4 : 0);
           const int nb = (int)min(64u, tslen - base);
-          for (int j = 0; j < nb; j++) {
+          // Only elements that can act are visited.  Within the loop ymax
+          // only falls and ymin never rises above the ymax of this point,
+          // so an element that can neither draw (needs bottom row s1 <
+          // ymax) nor stop the column (needs top row s2 <= ymin) from here
+          // on stays inert: the loop below would reach one of its
+          // `continue`s for it.
+          const bool can_act = lane < nb && (eflags & 3) == 3 &&
+                               (es2 <= ymax || ((eflags & 4) && es1 < ymax));
+          for (uint64_t act = __ballot(can_act); act != 0; act &= act - 1) {
+            const int j = (int)__builtin_ctzll(act);
             const int f = readlane_i(eflags, j);
             if (!(f & 1) || !(f & 2)) continue;
             int s2 = readlane_i(es2, j);
