@@ -349,6 +349,16 @@ def bench_c3(ctx, rlevox, workloads, frames=8, reps=3):
            "raycast_ms_mean": round(ray_ms, 3),
            "raycast_bytes_per_launch": round(mean_bytes),
            "raycast_gbs": round(mean_bytes / (ray_ms * 1e-3) / 1e9, 2)}
+    # measured HBM traffic and L2 hit rate of the same workload (rocprofv3
+    # FETCH_SIZE / WRITE_SIZE / TCC passes, copy-kernel calibrated)
+    prof = os.path.join("profiles", "k_raycast_wave_c3_4k_grazing.txt")
+    pj = profile_json(prof)
+    if pj.get("traffic_bytes"):
+        out["hbm_traffic_per_launch"] = pj["traffic_bytes"]
+        out["hbm_gbs"] = round(pj["traffic_bytes"] / (ray_ms * 1e-3) / 1e9, 2)
+        out["hbm_frac"] = round(out["hbm_gbs"] / HBM_PEAK_GBS, 5)
+        out["l2_hit"] = pj.get("l2_hit")
+        out["traffic_source"] = prof
     log("c3: %s" % out)
     for b in (d_rb, d_clip, d_c0, d_col, d_dep):
         b.free()
