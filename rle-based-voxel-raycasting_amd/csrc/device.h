@@ -564,17 +564,23 @@ __device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ c
         if (o[0] >= -kGapHalo) {
           // r_q8 >= 42 means dmin <= 1874: a tap qualifies when
           // 10*dmin >= 7*d, i.e. d <= floor(10*dmin/7) (which is below the sky depth)
+          // Branch-free: every tap's depth and colour are read (so all 50
+          // LDS reads can be in flight together) and a failing tap adds 0.
           int n = 1, sum = c0 & 0xff;
           const int dlim = (10 * dmin) / 7;
+          int rowo[5];
+#pragma unroll
+          for (int b = 0; b < 5; b++) rowo[b] = (ly - o[b]) * kGapTW + lx;
 #pragma unroll
           for (int a = 0; a < 5; a++) {
 #pragma unroll
             for (int b = 0; b < 5; b++) {
-              const int d = t.d[ly - o[b]][lx + o[a]];
-              if (d <= dlim) {
-                n++;
-                sum += t.c[ly - o[b]][lx + o[a]] & 0xff;
-              }
+              const int i = rowo[b] + o[a];
+              const int d = (&t.d[0][0])[i];
+              const int c = (&t.c[0][0])[i] & 0xff;
+              const bool q = d <= dlim;
+              n += q ? 1 : 0;
+              sum += q ? c : 0;
             }
           }
           if (n > 6) out = (uint16_t)((c0 & 0xff00) | ((sum + n / 2) / n));
