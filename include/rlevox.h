@@ -274,7 +274,9 @@ rlevox_status rlevox_raycast_counters(rlevox_ctx* ctx, const rlevox_scene* scene
                                       int32_t* d_line_clip, rlevox_counters* out);
 
 /* Compose (colorize_buddha_soft.frag:12-92): ray buffer -> W*H colour16 +
- * int32 depth, row 0 at the top of the image. */
+ * int32 depth, row 0 at the top of the image.  d_raybuf holds
+ * map_line_count*rs texels (any texel may be read; only those inside a
+ * line's cleared range are used). */
 rlevox_status rlevox_compose(rlevox_ctx* ctx, const rlevox_raymap* rm, const uint32_t* d_raybuf,
                              const int32_t* d_line_clip, uint16_t* d_colour,
                              int32_t* d_depth, void* stream);

@@ -443,14 +443,21 @@ __device__ inline void compose_pixel(const rlevox_raymap& rm, const uint32_t* __
     tc = min(max(tc, 0), rs - 1);
     tr = min(max(tr, 0), 4 * rs - 1);
     if (tr < rm.map_line_count) {
-      const int2 lc = *reinterpret_cast<const int2*>(line_clip + 2 * tr);
-      if (tc >= lc.x && tc <= lc.y) {
-        const uint32_t t = raybuf[(size_t)tr * (size_t)rs + (size_t)tc];
-        if (((t >> 16) & 0xffu) != 0xffu) {
-          c = (uint16_t)(t & 0xffffu);
-          d = (int32_t)(t >> 16);
-        }
-      }
+      // Both loads go out together: the texel is inside the ray buffer
+      // (map_line_count x rs) for any tc, and is used only inside the
+      // line's cleared range.
+      // Read as global memory: the batch kernels take these pointers from
+      // an LDS copy of the view, where the compiler would emit flat loads
+      // and wait on both counters between them.
+      typedef __attribute__((address_space(1))) const uint64_t g_u64;
+      typedef __attribute__((address_space(1))) const uint32_t g_u32;
+      const uint32_t t = *((g_u32*)raybuf + (size_t)tr * (size_t)rs + (size_t)tc);
+      const uint64_t lcw = *(g_u64*)(line_clip + 2 * tr);  // {lo, hi}, 8-byte aligned
+      const int2 lc = make_int2((int)(uint32_t)lcw, (int)(uint32_t)(lcw >> 32));
+      // non-short-circuit test: no branch for the compiler to sink a load into
+      const bool use = (tc >= lc.x) & (tc <= lc.y) & (((t >> 16) & 0xffu) != 0xffu);
+      c = use ? (uint16_t)(t & 0xffffu) : c;
+      d = use ? (int32_t)(t >> 16) : d;
     }
   }
   out_c = c;
