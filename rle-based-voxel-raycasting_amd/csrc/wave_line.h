@@ -682,9 +682,14 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     const uint16_t* cached = rc + lane * kRunCache + rc_delta;
     const int ccap = cand ? kRunCache - rc_delta : 0;
     static_assert(kRunCache == 40, "the run cache is fetched as five 16-byte words");
-    uint4 rc0, rc1, rc2, rc3, rc4;  // loaded and stored only for candidates
+    typedef unsigned int rc_word __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(1))) const rc_word g_rc_word;
+    rc_word rc0, rc1, rc2, rc3, rc4;  // loaded and stored only for candidates
     if (cand) {
-      const uint4* src = reinterpret_cast<const uint4*>(slab + rc_base);
+      // Global loads: in the drawing wavefront of a two-wavefront line the
+      // slab pointer comes through LDS, and flat loads would make each LDS
+      // wait until they return wait for them too.
+      const g_rc_word* src = (const g_rc_word*)(slab + rc_base);
       rc0 = src[0];
       rc1 = src[1];
       rc2 = src[2];
@@ -699,7 +704,7 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     // above, a drawing-only one after the span parameters below.
     auto store_rc = [&]() {
       if (cand) {
-        uint4* dst = reinterpret_cast<uint4*>(rc + lane * kRunCache);
+        rc_word* dst = reinterpret_cast<rc_word*>(rc + lane * kRunCache);
         dst[0] = rc0;
         dst[1] = rc1;
         dst[2] = rc2;
