@@ -528,16 +528,35 @@ __device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ c
   constexpr int kStage = (kGapTW * kGapTH + 255) / 256;
   int32_t dv[kStage];
   uint16_t cv[kStage];
+  // uniform: a tile whose border lies inside the image needs no clamping
+  const bool inner = x0 >= kGapHalo && y0 >= kGapHalo && x0 - kGapHalo + kGapTW <= W &&
+                     y0 - kGapHalo + kGapTH <= H;
+  if (inner) {
+    const size_t g0 = (size_t)(y0 - kGapHalo) * W + (size_t)(x0 - kGapHalo);
+    const int32_t* dt = dep + g0;
+    const uint16_t* ct = cin + g0;
 #pragma unroll
-  for (int k = 0; k < kStage; k++) {
-    const int i = threadIdx.x + k * 256;
-    if (i < kGapTW * kGapTH) {
-      const int ty = i / kGapTW, tx = i - ty * kGapTW;
-      const int gx = min(max(x0 - kGapHalo + tx, 0), W - 1);
-      const int gy = min(max(y0 - kGapHalo + ty, 0), H - 1);
-      const size_t g = (size_t)gy * W + gx;
-      dv[k] = dep[g];
-      cv[k] = cin[g];
+    for (int k = 0; k < kStage; k++) {
+      const int i = threadIdx.x + k * 256;
+      if (i < kGapTW * kGapTH) {
+        const int ty = i / kGapTW, tx = i - ty * kGapTW;
+        const int o = ty * W + tx;  // < kGapTH * W
+        dv[k] = dt[o];
+        cv[k] = ct[o];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kStage; k++) {
+      const int i = threadIdx.x + k * 256;
+      if (i < kGapTW * kGapTH) {
+        const int ty = i / kGapTW, tx = i - ty * kGapTW;
+        const int gx = min(max(x0 - kGapHalo + tx, 0), W - 1);
+        const int gy = min(max(y0 - kGapHalo + ty, 0), H - 1);
+        const size_t g = (size_t)gy * W + gx;
+        dv[k] = dep[g];
+        cv[k] = cin[g];
+      }
     }
   }
 #pragma unroll
