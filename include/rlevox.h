@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RLEVOX_ABI_VERSION 1
+#define RLEVOX_ABI_VERSION 2
 #define RLEVOX_MAX_MIPS 16
 /* Cleared ray-buffer texel (Cuda_Render.h:257). Its byte 2 (0xff) marks sky. */
 #define RLEVOX_CLEAR_TEXEL 0x00ff8844u
@@ -180,6 +180,12 @@ typedef struct {
   int32_t persistent_waves; /* wavefronts per CU of the persistent batch kernel
                                (0 = default 16, else 4..32) */
   int32_t order;           /* 0 = view order, 1 = longest views first */
+  int32_t run_cache;       /* LDS run-cache words per candidate column of the
+                              wavefront-per-line kernels (24, 40 or 56; 0 =
+                              40 with two wavefronts per line, 56 with one):
+                              deeper holds more of a column's runs and
+                              colours in LDS, shallower loads less per
+                              column.  The persistent batch kernel uses 40. */
 } rlevox_tuning;
 
 typedef struct {

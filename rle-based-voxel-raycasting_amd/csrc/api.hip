@@ -193,7 +193,8 @@ rlevox_status rlevox_render_raybuf(rlevox_ctx* ctx, const rlevox_scene* scene,
     t = ctx->tuning_for(scene, rm->W, rm->H);
   }
   hipError_t e = launch_raycast_plain(scene->dev, *rm, d_raybuf, d_line_clip,
-                                      ctx->tuning_block(t, rm->map_line_count), (hipStream_t)stream);
+                                      ctx->tuning_block(t, rm->map_line_count), t.run_cache,
+                                      (hipStream_t)stream);
   return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "raycast launch");
 }
 
@@ -214,7 +215,7 @@ rlevox_status rlevox_raycast_counters(rlevox_ctx* ctx, const rlevox_scene* scene
   if (e == hipSuccess)
     e = launch_raycast_plain(scene->dev, *rm, d_raybuf, d_line_clip,
                              ctx->tuning_block(ctx->tuning_for(scene, rm->W, rm->H), rm->map_line_count),
-                             nullptr, d);
+                             0, nullptr, d);
   if (e == hipSuccess) e = hipMemcpy(out, d, sizeof(rlevox_counters), hipMemcpyDeviceToHost);
   return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "raycast_counters");
 }
@@ -271,8 +272,9 @@ rlevox_status rlevox_render(rlevox_ctx* ctx, const rlevox_scene* scene, const rl
   hipStream_t s = (hipStream_t)stream;
   uint32_t* raybuf = (uint32_t*)ctx->raybuf.ptr;
   int32_t* clip = (int32_t*)ctx->clip.ptr;
+  const rlevox_tuning t = ctx->tuning_for(scene, W, H);
   hipError_t e = launch_raycast_plain(scene->dev, rm, raybuf, clip,
-                                      ctx->tuning_block(ctx->tuning_for(scene, W, H), rm.map_line_count), s);
+                                      ctx->tuning_block(t, rm.map_line_count), t.run_cache, s);
   if (e != hipSuccess) return hip_fail(e, "raycast launch");
   uint16_t* cmp = tb ? (uint16_t*)ctx->ctmp.ptr : d_colour;
   e = launch_compose(rm, raybuf, clip, cmp, d_depth, s);

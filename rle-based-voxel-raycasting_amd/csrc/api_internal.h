@@ -29,7 +29,8 @@ inline bool tuning_valid(const rlevox_tuning& t) {
           t.lines_per_wave == 64) &&
          t.waves_per_block >= (t.lines_per_wave == 1 ? 0 : 1) && t.waves_per_block <= 4 &&
          (t.persistent_waves == 0 || (t.persistent_waves >= 4 && t.persistent_waves <= 32)) &&
-         (t.order == 0 || t.order == 1);
+         (t.order == 0 || t.order == 1) &&
+         (t.run_cache == 0 || t.run_cache == 24 || t.run_cache == 40 || t.run_cache == 56);
 }
 
 inline int post_flags(const rlevox_render_params* p) {
@@ -100,7 +101,7 @@ struct rlevox_ctx {
   std::mutex mu;  // guards the scratch buffers and tuning
   rlevox::Scratch raybuf, clip, ctmp, counts;
   rlevox::BatchSlot batch;
-  rlevox_tuning tuning{1, 0, 0, 1};
+  rlevox_tuning tuning{1, 0, 0, 1, 0};
   // Autotuned settings per (scene, W, H); others use `tuning`.
   struct TuneKey {
     const rlevox_scene* scene;

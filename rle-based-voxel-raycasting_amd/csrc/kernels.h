@@ -7,8 +7,12 @@
 
 namespace rlevox {
 
+// block: 1 = wavefront per line, 2 = two wavefronts per line, else lanes per
+// block of the lane-per-line kernel.  run_cache: LDS run-cache words of the
+// wavefront kernels (24, 40 or 56; 0 = 40 with two wavefronts per line, 56
+// with one; the counting variant always uses 40).
 hipError_t launch_raycast_plain(const SceneDev& sc, const rlevox_raymap& rm, uint32_t* raybuf,
-                                int32_t* clip, int block, hipStream_t s,
+                                int32_t* clip, int block, int run_cache, hipStream_t s,
                                 unsigned long long* counts = nullptr);
 hipError_t launch_compose(const rlevox_raymap& rm, const uint32_t* raybuf, const int32_t* clip,
                           uint16_t* col, int32_t* dep, hipStream_t s);

@@ -33,35 +33,40 @@ __global__ __launch_bounds__(256) void k_raycast_plain(SceneDev sc, rlevox_rayma
 // One wavefront (= one block) per ray line.  Registers are capped for four
 // wavefronts per SIMD (4096 resident lines on 256 CUs); uncapped the
 // compiler takes 140 and only three fit.
-template <bool COUNT>
+template <bool COUNT, int RC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_raycast_wave(SceneDev sc, rlevox_raymap rm,
                                                      uint32_t* __restrict__ raybuf,
                                                      int32_t* __restrict__ clip,
                                                      unsigned long long* counts) {
-  __shared__ uint4 rcache[64 * kRunCache / 8];
+  static_assert(kDdaLds * 4 <= 64 * RC * 2, "DDA scratch must fit in the run cache");
+  __shared__ uint4 rcache[64 * RC / 8];
   __shared__ int sweep[64];
   float* dda = reinterpret_cast<float*>(rcache);  // see kDdaLds
   extern __shared__ uint32_t lds[];
   if ((int)blockIdx.x >= rm.map_line_count) return;
   const int x = rm.map_line_count - 1 - (int)blockIdx.x;  // last line first
-  render_line_wave<COUNT>(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs, clip + 2 * x, lds, dda,
-                          reinterpret_cast<uint16_t*>(rcache), sweep, counts);
+  render_line_wave<COUNT, false, true, RC>(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs,
+                                           clip + 2 * x, lds, dda,
+                                           reinterpret_cast<uint16_t*>(rcache), sweep, counts);
 }
 
 // Two wavefronts (= one block) per ray line: wavefront 0 walks the DDA and
 // gathers the pointer map, wavefront 1 projects, caches and draws.
+template <int RC>
 __global__ __launch_bounds__(128) void k_raycast_wave2(SceneDev sc, rlevox_raymap rm,
                                                        uint32_t* __restrict__ raybuf,
                                                        int32_t* __restrict__ clip) {
-  __shared__ uint4 rcache[64 * kRunCache / 8];
+  __shared__ uint4 rcache[64 * RC / 8];
   __shared__ int sweep[64];
   __shared__ float dda[kDdaLds];
   __shared__ ChunkRing ring;
   extern __shared__ uint32_t lds[];
   if ((int)blockIdx.x >= rm.map_line_count) return;
   const int x = rm.map_line_count - 1 - (int)blockIdx.x;  // last line first
-  render_line_wave<false, true>(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs, clip + 2 * x, lds,
-                                dda, reinterpret_cast<uint16_t*>(rcache), sweep, nullptr, &ring);
+  render_line_wave<false, true, true, RC>(x, sc, rm, raybuf + (size_t)x * (size_t)rm.rs,
+                                          clip + 2 * x, lds, dda,
+                                          reinterpret_cast<uint16_t*>(rcache), sweep, nullptr,
+                                          &ring);
 }
 
 __global__ __launch_bounds__(256) void k_compose(rlevox_raymap rm, const uint32_t* __restrict__ raybuf,
@@ -85,27 +90,40 @@ __global__ __launch_bounds__(256) void k_gapfill(int W, int H, const uint16_t* _
   gapfill_tile(W, H, cin, dep, cout, blockIdx.x * 64, blockIdx.y * 4, tile);
 }
 
+template <int RC>
+static void launch_wave(const SceneDev& sc, const rlevox_raymap& rm, uint32_t* raybuf,
+                        int32_t* clip, bool two, hipStream_t s) {
+  const size_t lds = (size_t)bitmap_words(rm.rs) * sizeof(uint32_t);
+  if (two)
+    hipLaunchKernelGGL(k_raycast_wave2<RC>, dim3(rm.map_line_count), dim3(128), lds, s, sc, rm,
+                       raybuf, clip);
+  else
+    hipLaunchKernelGGL((k_raycast_wave<false, RC>), dim3(rm.map_line_count), dim3(64), lds, s, sc,
+                       rm, raybuf, clip, nullptr);
+}
+
 hipError_t launch_raycast_plain(const SceneDev& sc, const rlevox_raymap& rm, uint32_t* raybuf,
-                                int32_t* clip, int block, hipStream_t s,
+                                int32_t* clip, int block, int run_cache, hipStream_t s,
                                 unsigned long long* counts) {
   const int lines = rm.map_line_count;
   if (lines <= 0) return hipSuccess;
-  if (block == 2) {  // two wavefronts per line (counting uses one)
-    const size_t lds = (size_t)bitmap_words(rm.rs) * sizeof(uint32_t);
-    if (!counts) {
-      hipLaunchKernelGGL(k_raycast_wave2, dim3(lines), dim3(128), lds, s, sc, rm, raybuf, clip);
-      return hipGetLastError();
+  if (block == 1 || block == 2) {  // wavefront kernels
+    // Default depth: 40 words with two wavefronts per line (1080p
+    // flythrough: 0.1504 ms vs 0.1509 / 0.1514 with 24 / 56), 56 with one,
+    // where frames have many long lines (4K flythrough 0.267 vs 0.272 ms,
+    // 4K grazing views of the 16384^2 terrain 8.03 vs 8.10 ms).
+    if (run_cache == 0) run_cache = block == 2 ? kRunCache : 56;
+    if (counts) {  // counting walks with one wavefront and the default run cache
+      const size_t lds = (size_t)bitmap_words(rm.rs) * sizeof(uint32_t);
+      hipLaunchKernelGGL((k_raycast_wave<true, kRunCache>), dim3(lines), dim3(64), lds, s, sc, rm,
+                         raybuf, clip, counts);
+    } else if (run_cache == 24) {
+      launch_wave<24>(sc, rm, raybuf, clip, block == 2, s);
+    } else if (run_cache == 56) {
+      launch_wave<56>(sc, rm, raybuf, clip, block == 2, s);
+    } else {
+      launch_wave<kRunCache>(sc, rm, raybuf, clip, block == 2, s);
     }
-    block = 1;
-  }
-  if (block == 1) {  // one wavefront per line
-    const size_t lds = (size_t)bitmap_words(rm.rs) * sizeof(uint32_t);
-    if (counts)
-      hipLaunchKernelGGL(k_raycast_wave<true>, dim3(lines), dim3(64), lds, s, sc, rm, raybuf, clip,
-                         counts);
-    else
-      hipLaunchKernelGGL(k_raycast_wave<false>, dim3(lines), dim3(64), lds, s, sc, rm, raybuf,
-                         clip, counts);
     return hipGetLastError();
   }
   const int grid = (lines + block - 1) / block;

@@ -14,10 +14,12 @@ using namespace rlevox;
 namespace {
 
 // Candidates: one or two wavefronts per line (lanes take consecutive DDA
-// steps), and one lane per line in blocks of 16..256 lanes.
+// steps) with an LDS run cache of 24, 40 or 56 words per column, and one
+// lane per line in blocks of 16..256 lanes.
 const rlevox_tuning kCandidates[] = {
-    {1, 1, 0, 1},  {1, 2, 0, 1},  {64, 1, 0, 1}, {64, 2, 0, 1}, {64, 4, 0, 1},
-    {32, 1, 0, 1}, {32, 2, 0, 1}, {16, 1, 0, 1}, {16, 4, 0, 1},
+    {1, 1, 0, 1, 40}, {1, 2, 0, 1, 40}, {1, 1, 0, 1, 24}, {1, 2, 0, 1, 24}, {1, 1, 0, 1, 56},
+    {1, 2, 0, 1, 56}, {64, 1, 0, 1},    {64, 2, 0, 1},    {64, 4, 0, 1},    {32, 1, 0, 1},
+    {32, 2, 0, 1},    {16, 1, 0, 1},    {16, 4, 0, 1},
 };
 
 rlevox_status time_views(rlevox_ctx* ctx, const rlevox_view* views, int n,

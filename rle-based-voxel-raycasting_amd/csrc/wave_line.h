@@ -65,6 +65,8 @@ __device__ inline void wave_sync() {
 
 // Slab words cached per step (from the 16-byte boundary at or below the
 // column header): covers header, elements and colours of typical columns.
+// The default depth; the wavefront-per-line kernels are also built for 24
+// and 56 words (a tuning choice).
 constexpr int kRunCache = 40;
 // LDS floats per wavefront used by the DDA chunk.  The kernels place them in
 // the run cache's memory: the DDA of chunk c+1 runs after chunk c's replay
@@ -449,7 +451,7 @@ __device__ inline void ring_get(const ChunkRing& r, int s, DdaChunk& c, int lane
 // precomputes a column's remaining elements across the lanes (pays where
 // steps often show several elements, as in grazing views of caved terrain;
 // the persistent batch kernel leaves it off, where its registers spill).
-template <bool COUNT, bool TWO = false, bool PAR_ELEMS = true>
+template <bool COUNT, bool TWO = false, bool PAR_ELEMS = true, int RC = kRunCache>
 __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_raymap& rm,
                                         uint32_t* __restrict__ row, int32_t* __restrict__ clip,
                                         uint32_t* bits, float* dl, uint16_t* rc, int* sweep_lds,
@@ -679,22 +681,20 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     const bool cand = valid && F > ymin && !culled0;
     const uint64_t rc_base = wofs & ~(uint64_t)7;  // 16-byte aligned start
     const int rc_delta = (int)(wofs - rc_base);
-    const uint16_t* cached = rc + lane * kRunCache + rc_delta;
-    const int ccap = cand ? kRunCache - rc_delta : 0;
-    static_assert(kRunCache == 40, "the run cache is fetched as five 16-byte words");
+    const uint16_t* cached = rc + lane * RC + rc_delta;
+    const int ccap = cand ? RC - rc_delta : 0;
+    static_assert(RC % 8 == 0 && RC >= 24, "the run cache is fetched in 16-byte words");
+    constexpr int kWords = RC / 8;
     typedef unsigned int rc_word __attribute__((ext_vector_type(4)));
     typedef __attribute__((address_space(1))) const rc_word g_rc_word;
-    rc_word rc0, rc1, rc2, rc3, rc4;  // loaded and stored only for candidates
+    rc_word rcw[kWords];  // loaded and stored only for candidates
     if (cand) {
       // Global loads: in the drawing wavefront of a two-wavefront line the
       // slab pointer comes through LDS, and flat loads would make each LDS
       // wait until they return wait for them too.
       const g_rc_word* src = (const g_rc_word*)(slab + rc_base);
-      rc0 = src[0];
-      rc1 = src[1];
-      rc2 = src[2];
-      rc3 = src[3];
-      rc4 = src[4];
+#pragma unroll
+      for (int i = 0; i < kWords; i++) rcw[i] = src[i];
     }
 
     // ---- next chunk's DDA (the run cache loads are in flight meanwhile)
@@ -704,12 +704,9 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     // above, a drawing-only one after the span parameters below.
     auto store_rc = [&]() {
       if (cand) {
-        rc_word* dst = reinterpret_cast<rc_word*>(rc + lane * kRunCache);
-        dst[0] = rc0;
-        dst[1] = rc1;
-        dst[2] = rc2;
-        dst[3] = rc3;
-        dst[4] = rc4;
+        rc_word* dst = reinterpret_cast<rc_word*>(rc + lane * RC);
+#pragma unroll
+        for (int i = 0; i < kWords; i++) dst[i] = rcw[i];
       }
     };
     if constexpr (decltype(produce)::value) store_rc();
@@ -847,7 +844,7 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
               const uint64_t wofs = (uint64_t)aofs << shift;
               const int tdelta = (int)(wofs & 7u);
               const int cw = 2 + (int)alen + tdelta + (int)u;
-              const uint32_t col = cw < kRunCache ? rc[a * kRunCache + cw]
+              const uint32_t col = cw < RC ? rc[a * RC + cw]
                                                   : slab[2 + wofs + alen + u];
               row[ymin + lane] = col + (depth << 16);
             }
@@ -886,8 +883,8 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
       const uint16_t* tcol = truns + tslen;
       const bool tcached = readlane_i((int)cand, t) != 0;
       const int tdelta = (int)(tofs & 7u);
-      const uint16_t* tcache = rc + t * kRunCache + tdelta;
-      const int tcap = tcached ? kRunCache - tdelta : 0;
+      const uint16_t* tcache = rc + t * RC + tdelta;
+      const int tcap = tcached ? RC - tdelta : 0;
       const float tpz = readlane_f(pz, t), tpy = readlane_f(py3, t);
       const float tczz = readlane_f(czz, t), tcyy = readlane_f(cyy, t);
 

@@ -98,10 +98,11 @@ class View(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("lines_per_wave", C.c_int32), ("waves_per_block", C.c_int32),
-                ("persistent_waves", C.c_int32), ("order", C.c_int32)]
+                ("persistent_waves", C.c_int32), ("order", C.c_int32), ("run_cache", C.c_int32)]
 
     def as_tuple(self):
-        return (self.lines_per_wave, self.waves_per_block, self.persistent_waves, self.order)
+        return (self.lines_per_wave, self.waves_per_block, self.persistent_waves, self.order,
+                self.run_cache)
 
 
 class ServerConfig(C.Structure):
@@ -418,8 +419,9 @@ class Context:
         check(lib.rlevox_event_elapsed_ms(self.handle, a, b, C.byref(ms)))
         return float(ms.value)
 
-    def set_tuning(self, lines_per_wave=1, waves_per_block=0, persistent_waves=0, order=1):
-        t = Tuning(lines_per_wave, waves_per_block, persistent_waves, order)
+    def set_tuning(self, lines_per_wave=1, waves_per_block=0, persistent_waves=0, order=1,
+                   run_cache=0):
+        t = Tuning(lines_per_wave, waves_per_block, persistent_waves, order, run_cache)
         check(lib.rlevox_set_tuning(self.handle, C.byref(t)))
 
     def get_tuning(self):

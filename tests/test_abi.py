@@ -53,7 +53,7 @@ def test_struct_layouts_match_c(tmp_path):
 
 
 def test_abi_version():
-    assert rlevox.lib.rlevox_abi_version() == 1
+    assert rlevox.lib.rlevox_abi_version() == 2
 
 
 def test_no_device_is_reported_not_crashed():

@@ -50,9 +50,11 @@ def assert_same(name, got, want):
                              % (name, len(diff), got.size, i, got[i], want[i]))
 
 
-# Raycast kernel variants (lines per wavefront): one lane per line in blocks
-# of 64 or 16 lanes, and one wavefront per line.  All must agree bit-exactly.
-KERNELS = [(64, 1), (16, 1), (1, 1), (1, 2)]
+# Raycast kernel variants (lines per wavefront, waves per block, LDS run-cache
+# words): one lane per line in blocks of 64 or 16 lanes, one wavefront per
+# line, two wavefronts per line, and the run-cache depths the tuning offers.
+# All must agree bit-exactly.
+KERNELS = [(64, 1, 0), (16, 1, 0), (1, 1, 40), (1, 2, 40), (1, 1, 24), (1, 2, 56)]
 
 
 def check_view(ctx, oracle, dscene, mips, pose, W, H, mip_distance=0, far_distance=0):
@@ -63,10 +65,10 @@ def check_view(ctx, oracle, dscene, mips, pose, W, H, mip_distance=0, far_distan
     ref_col, ref_dep = oracle.compose(orm, ref_rb, ref_clip)
     ref_fin = oracle.gapfill(ref_col, ref_dep)
     n = max(rm.map_line_count, 1)
-    for lpw, wpb in KERNELS:
-        ctx.set_tuning(lines_per_wave=lpw, waves_per_block=wpb)
+    for lpw, wpb, rc in KERNELS:
+        ctx.set_tuning(lines_per_wave=lpw, waves_per_block=wpb, run_cache=rc)
         got = gpu_stages(ctx, dscene, rm)
-        tag = " [lines/wave %d, waves/block %d]" % (lpw, wpb)
+        tag = " [lines/wave %d, waves/block %d, run cache %d]" % (lpw, wpb, rc)
         assert_same("line clip" + tag, got["clip"][:n], ref_clip[:n])
         assert_same("ray buffer" + tag, got["raybuf"], ref_rb)
         assert_same("compose colour" + tag, got["compose"], ref_col)
@@ -152,8 +154,8 @@ def test_work_counters_match_oracle(ctx, oracle, c1_scene, c1_dev):
         d_rb = ctx.alloc(lines * rm.rs * 4)
         d_clip = ctx.alloc(lines * 8)
         _, _, want = oracle.raycast(c1_scene.mips, oracle.raymap(p, 1280, 720))
-        for lpw, wpb in KERNELS:
-            ctx.set_tuning(lines_per_wave=lpw, waves_per_block=wpb)
+        for lpw, wpb, rc in KERNELS:
+            ctx.set_tuning(lines_per_wave=lpw, waves_per_block=wpb, run_cache=rc)
             got = ctx.raycast_counters(c1_dev, rm, d_rb, d_clip)
             assert got == want, (lpw, got, want)
         ctx.set_tuning()
@@ -193,6 +195,15 @@ def test_offset_shift_scene(ctx, oracle, c1_scene, shift):
             check_view(ctx, oracle, dev, c1_scene.mips, p, 512, 384)
     finally:
         dev.free()
+
+
+def test_tuning_rejects_other_run_cache_depths(ctx):
+    for bad in (8, 32, 48, 64):
+        with pytest.raises(rlevox.RlevoxError):
+            ctx.set_tuning(run_cache=bad)
+    ctx.set_tuning(run_cache=56)
+    assert ctx.get_tuning().run_cache == 56
+    ctx.set_tuning()
 
 
 @pytest.mark.parametrize("W,H", [(333, 97), (64, 16), (1000, 17), (1920, 1080)])
