@@ -158,6 +158,15 @@ rlevox_status rlevox_scene_upload(rlevox_ctx* ctx, const rlevox_mip* mips, int32
     return hip_fail(e, "scene_upload");
   }
   s->d_dev = (SceneDev*)dd;
+  {  // runs per level-0 column, from up to 65536 evenly spaced columns
+    const rlevox_mip& m0 = m[0];
+    const uint64_t cols = (uint64_t)m0.sx * (uint64_t)m0.sz;
+    const uint64_t step = std::max<uint64_t>(1, cols / 65536);
+    uint64_t runs = 0, k = 0;
+    for (uint64_t c = 0; c < cols; c += step, k++)
+      runs += m0.slabs[(uint64_t)m0.ptrmap[2 * c] << m0.offset_shift];
+    s->mean_runs = k ? (float)((double)runs / (double)k) : 0.0f;
+  }
   *out = s;
   return RLEVOX_OK;
 }

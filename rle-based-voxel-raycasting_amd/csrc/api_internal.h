@@ -80,7 +80,9 @@ struct BatchSlot {
   hipGraphExec_t exec = nullptr;
   int nviews = 0;
   int waves = kWavesPerCU;        // persistent wavefronts per CU of this batch
-  int graph_waves = 0;            // the value the captured graph was made with
+  bool par_elems = false;         // raycast with the lane-parallel element replay
+  int graph_waves = 0;            // the values the captured graph was made with
+  bool graph_par = false;
   uint64_t rays = 0;
   rlevox_status init(rlevox_ctx* c);
   rlevox_status prepare(const rlevox_view* views, int n, const rlevox_render_params* params,
@@ -153,4 +155,5 @@ struct rlevox_scene {
   rlevox::SceneDev* d_dev = nullptr;  // device copy, read by batched kernels
   std::vector<void*> allocs;
   uint64_t bytes = 0;
+  float mean_runs = 0;  // runs per level-0 column, from a sample taken at upload
 };

@@ -32,7 +32,9 @@ struct BatchHeader {
   uint32_t pad;
 };
 
+// par_elems: the raycast replays a column's elements with the lane-parallel
+// preparation (the plain kernels' replay) instead of one by one.
 hipError_t launch_batch(BatchHeader* d_hdr, const ViewDesc* d_views, int raycast_grid, int post_grid,
-                        hipStream_t s);
+                        bool par_elems, hipStream_t s);
 
 }  // namespace rlevox

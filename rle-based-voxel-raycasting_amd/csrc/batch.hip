@@ -38,7 +38,12 @@ __device__ inline int find_view(const ViewDesc* __restrict__ v, int n, int64_t g
 // Register use is capped for four wavefronts per SIMD (16 per CU, the
 // default launch): a few cold values spill, and the batch still runs 5%
 // faster than at the three wavefronts per SIMD its natural 152 registers
-// allow.
+// allow.  PAR: the replay prepares a column's elements across the lanes and
+// visits only those that can act, as the plain kernels do; it spills more
+// and pays only where columns hold many runs (served mix with three 16384^2
+// scenes resident: 4165 -> 4769 Mrays/s; 64 views over 4096^2 terrains:
+// 10.36 -> 10.86 ms), so a batch takes it by its scenes' runs per column.
+template <bool PAR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_raycast_batch(BatchHeader* __restrict__ h,
                                                       const ViewDesc* __restrict__ views) {
   __shared__ uint32_t bits[(RLEVOX_MAX_RS + 31) / 32];
@@ -58,7 +63,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const ViewDesc& vd = views[v];
     // lines of a view from the last to the first, as the single-view kernels
     const int x = vd.rm.map_line_count - 1 - (int)((int64_t)g - vd.line_begin);
-    render_line_wave<false, false, false>(x, *vd.scene, vd.rm,
+    render_line_wave<false, false, PAR>(x, *vd.scene, vd.rm,
                                           vd.raybuf + (size_t)x * (size_t)vd.rm.rs,
                             vd.clip + 2 * x, bits, dda, reinterpret_cast<uint16_t*>(rcache),
                             sweep, nullptr);
@@ -127,9 +132,12 @@ __global__ __launch_bounds__(256) void k_gapfill_batch(const BatchHeader* __rest
 }
 
 hipError_t launch_batch(BatchHeader* d_hdr, const ViewDesc* d_views, int raycast_grid, int post_grid,
-                        hipStream_t s) {
+                        bool par_elems, hipStream_t s) {
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, d_hdr);
-  hipLaunchKernelGGL(k_raycast_batch, dim3(raycast_grid), dim3(64), 0, s, d_hdr, d_views);
+  if (par_elems)
+    hipLaunchKernelGGL(k_raycast_batch<true>, dim3(raycast_grid), dim3(64), 0, s, d_hdr, d_views);
+  else
+    hipLaunchKernelGGL(k_raycast_batch<false>, dim3(raycast_grid), dim3(64), 0, s, d_hdr, d_views);
   hipLaunchKernelGGL(k_compose_batch, dim3(post_grid), dim3(256), 0, s, d_hdr, d_views);
   hipLaunchKernelGGL(k_gapfill_batch, dim3(post_grid), dim3(256), 0, s, d_hdr, d_views);
   return hipGetLastError();
@@ -184,6 +192,12 @@ rlevox_status BatchSlot::prepare(const rlevox_view* views, int n, const rlevox_r
   waves = tuning.persistent_waves ? tuning.persistent_waves : kWavesPerCU;
   if (tuning.order == 1)
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  // Lane-parallel element replay when some scene's columns hold more than
+  // kParRuns runs on average (the generated 4096^2 terrains: 5.6-5.9; the
+  // 16384^2 terrain's settings give 7.6 already at 4096^2).
+  constexpr float kParRuns = 7.0f;
+  par_elems = false;
+  for (int i = 0; i < n; i++) par_elems = par_elems || views[i].scene->mean_runs > kParRuns;
   // scratch layout: per view ray buffer, line clip ranges, composed colour
   size_t bytes = 0;
   std::vector<size_t> off_rb(n), off_clip(n), off_ct(n);
@@ -259,10 +273,10 @@ rlevox_status BatchSlot::run(bool use_graph, hipStream_t s) {
   const int ray_grid = cus * waves;
   const int post_grid = cus * 8;
   if (!use_graph) {
-    e = launch_batch(d_hdr, d_views, ray_grid, post_grid, s);
+    e = launch_batch(d_hdr, d_views, ray_grid, post_grid, par_elems, s);
     return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "batch launch");
   }
-  if (exec && graph_waves != waves) {  // launch shape changed: capture again
+  if (exec && (graph_waves != waves || graph_par != par_elems)) {  // kernels changed: capture again
     hipGraphExecDestroy(exec);
     hipGraphDestroy(graph);
     exec = nullptr;
@@ -270,11 +284,12 @@ rlevox_status BatchSlot::run(bool use_graph, hipStream_t s) {
   }
   if (!exec) {
     graph_waves = waves;
+    graph_par = par_elems;
     hipStream_t cap = nullptr;
     e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      hipError_t le = launch_batch(d_hdr, d_views, ray_grid, post_grid, cap);
+      hipError_t le = launch_batch(d_hdr, d_views, ray_grid, post_grid, par_elems, cap);
       e = hipStreamEndCapture(cap, &graph);
       if (e == hipSuccess) e = le;
     }
