@@ -397,16 +397,16 @@ struct OrcScene {
   int nmips;
 };
 
-static void orc_render_line(int x, const OrcScene& sc, const rlevox_raymap& rm, uint32_t* raybuf,
-                            int32_t* line_clip, uint32_t* y_cache, rlevox_counters* ctr) {
-  const int res_x = rm.rs, res_y = rm.rs;
-  const float res_x2 = (float)(res_x / 2);
-  const float res_y2 = (float)(res_y / 2);
-  uint32_t* out = raybuf + (size_t)x * (size_t)res_y;
-  line_clip[2 * x] = 0;
-  line_clip[2 * x + 1] = -1;
+// The plane of ray line x (Cuda_Render.h:107-172): its horizontal ray
+// direction in the camera's yaw frame, whether it belongs to an upper/lower
+// segment (`vertical`), and its clipped 2D end points.
+struct LinePlane {
+  float ray_x, ray_z;
+  bool vertical;
+  V3 start2d, end2d;
+};
 
-  // Line plane (Cuda_Render.h:128-172)
+static LinePlane line_plane(int x, const rlevox_raymap& rm) {
   int rays[4];
   rays[0] = rm.res[0];
   rays[1] = rm.res[1] + rays[0];
@@ -436,10 +436,28 @@ static void orc_render_line(int x, const OrcScene& sc, const rlevox_raymap& rm, 
     delta = delta * inv_len;
   }
   // vec3f_rot_y(viewrot.y) (Cuda_Render.h:75-80)
-  float ray_x = rm.cos_myaw * delta.x + rm.sin_myaw * delta.z;
-  float ray_z = rm.cos_myaw * delta.z - rm.sin_myaw * delta.x;
-  const V3 start2d = p1, end2d = p2;
-  const bool vertical = (1 - (j >> 1)) != 0;
+  LinePlane lp;
+  lp.ray_x = rm.cos_myaw * delta.x + rm.sin_myaw * delta.z;
+  lp.ray_z = rm.cos_myaw * delta.z - rm.sin_myaw * delta.x;
+  lp.start2d = p1;
+  lp.end2d = p2;
+  lp.vertical = (1 - (j >> 1)) != 0;
+  return lp;
+}
+
+static void orc_render_line(int x, const OrcScene& sc, const rlevox_raymap& rm, uint32_t* raybuf,
+                            int32_t* line_clip, uint32_t* y_cache, rlevox_counters* ctr) {
+  const int res_x = rm.rs, res_y = rm.rs;
+  const float res_x2 = (float)(res_x / 2);
+  const float res_y2 = (float)(res_y / 2);
+  uint32_t* out = raybuf + (size_t)x * (size_t)res_y;
+  line_clip[2 * x] = 0;
+  line_clip[2 * x + 1] = -1;
+
+  const LinePlane lp = line_plane(x, rm);
+  const float ray_x = lp.ray_x, ray_z = lp.ray_z;
+  const V3 start2d = lp.start2d, end2d = lp.end2d;
+  const bool vertical = lp.vertical;
 
   // Render vars (Cuda_Render.h:177-191)
   int mip_lvl = 0;
@@ -459,7 +477,11 @@ static void orc_render_line(int x, const OrcScene& sc, const rlevox_raymap& rm, 
   if (vertical) res_x2_mul_reverse = -res_x2_mul_reverse;
 
   // Screen-space clipping (Cuda_Render.h:215-250)
+#if ORC_PLANT == 8
+  int p_add = reverse ? -2 : 1;
+#else
   int p_add = reverse ? 1 : -2;
+#endif
   int p1x = f2i((float)res_x * start2d.x) + p_add;
   int p1y = f2i((float)res_y * start2d.y) + p_add;
   int p2x = f2i((float)res_x * end2d.x) - p_add;
@@ -662,7 +684,11 @@ static void orc_render_line(int x, const OrcScene& sc, const rlevox_raymap& rm, 
       float pos3d_z2 = correct_zz2 + pos3d_z_add * sg;
       if (pos3d_z2 <= 0) continue;
       float pos3d_y2 = correct_yy2 + pos3d_y_add * sg;
+#if ORC_PLANT == 6
+      int scr_y1 = f2i(res_y2 + pos3d_y2 / pos3d_z2);
+#else
       int scr_y1 = f2i(res_y2 + pos3d_y2 / pos3d_z2 - 1.0f);
+#endif
       if (scr_y1 >= y_clip_max) continue;
 
       // Floating horizon (Cuda_Render.h:563-580)
@@ -703,9 +729,15 @@ static void orc_render_line(int x, const OrcScene& sc, const rlevox_raymap& rm, 
         float fu = std::fmin(std::fmax(uz / onez, (float)texture), tex - 1.0f);
 #if ORC_PLANT == 1
         fu = (float)texture;
+#elif ORC_PLANT == 7
+        fu = std::fmin(std::fmax(uz / onez, (float)texture), tex);
 #endif
         uint32_t u = (uint32_t)f2i(fu);
+#if ORC_PLANT == 9
+        uint32_t real_z = (uint32_t)f2i(1.0f / onez + 0.5f) & 0xfffeu;
+#else
         uint32_t real_z = (uint32_t)f2i(1.0f / onez) & 0xfffeu;
+#endif
         uint32_t color16 = send[u];
         out[y] = color16 + (real_z << 16);
         c_col++;
@@ -816,7 +848,11 @@ static inline void compose_pixel(const rlevox_raymap& rm, const uint32_t* raybuf
   int32_t d = RLEVOX_SKY_DEPTH;
   if (std::isfinite(tx) && std::isfinite(ty)) {
     int tc = f2i(std::floor(tx * (float)rs));
+#if ORC_PLANT == 10
+    int tr = f2i(std::floor(ty * (float)(4 * rs) + 0.5f));
+#else
     int tr = f2i(std::floor(ty * (float)(4 * rs)));
+#endif
     tc = tc < 0 ? 0 : (tc > rs - 1 ? rs - 1 : tc);
     tr = tr < 0 ? 0 : (tr > 4 * rs - 1 ? 4 * rs - 1 : tr);
     if (tr < rm.map_line_count && tc >= line_clip[2 * tr] && tc <= line_clip[2 * tr + 1]) {
@@ -1067,5 +1103,22 @@ int orc_decode(const rlevox_mip* mip, uint8_t* surf, uint16_t* col) {
 }
 
 int orc_plant(void) { return ORC_PLANT; }
+
+// Line-plane parameters of every ray line, for checks of the per-line
+// arithmetic downstream of them: out[7*x ..] = ray_x, ray_z, vertical (0/1),
+// start2d.x, start2d.y, end2d.x, end2d.y.
+void orc_line_planes(const rlevox_raymap* rm, float* out) {
+  for (int x = 0; x < rm->map_line_count; x++) {
+    const LinePlane lp = line_plane(x, *rm);
+    float* o = out + 7 * (size_t)x;
+    o[0] = lp.ray_x;
+    o[1] = lp.ray_z;
+    o[2] = lp.vertical ? 1.0f : 0.0f;
+    o[3] = lp.start2d.x;
+    o[4] = lp.start2d.y;
+    o[5] = lp.end2d.x;
+    o[6] = lp.end2d.y;
+  }
+}
 
 }  // extern "C"

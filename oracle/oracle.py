@@ -56,6 +56,8 @@ class Oracle:
         L.orc_brute_render.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, C.POINTER(Pose), C.c_int,
                                        C.c_int, C.c_double, vp, vp, vp]
         L.orc_plant.restype = C.c_int
+        L.orc_line_planes.restype = None
+        L.orc_line_planes.argtypes = [C.POINTER(RayMap), vp]
         self.plant = L.orc_plant()
 
     @staticmethod
@@ -78,6 +80,12 @@ class Oracle:
         self.lib.orc_raycast(arr, len(mips), C.byref(rm), self._p(raybuf), self._p(clip),
                              C.byref(ctr), threads)
         return raybuf, clip, ctr.as_dict()
+
+    def line_planes(self, rm):
+        """Per ray line: ray_x, ray_z, vertical, start2d.xy, end2d.xy."""
+        out = np.zeros((max(rm.map_line_count, 1), 7), np.float32)
+        self.lib.orc_line_planes(C.byref(rm), self._p(out))
+        return out[:rm.map_line_count]
 
     def compose(self, rm, raybuf, clip):
         col = np.zeros((rm.H, rm.W), np.uint16)
