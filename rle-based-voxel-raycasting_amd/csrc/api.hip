@@ -84,6 +84,7 @@ rlevox_status rlevox_ctx_create(int32_t device, rlevox_ctx** out) {
   rlevox_ctx* c = new rlevox_ctx();
   c->device = device;
   c->num_cus = prop.multiProcessorCount;
+  c->max_lds = prop.sharedMemPerBlock;
   std::snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
   *out = c;
   return RLEVOX_OK;
@@ -202,7 +203,7 @@ rlevox_status rlevox_render_raybuf(rlevox_ctx* ctx, const rlevox_scene* scene,
     t = ctx->tuning_for(scene, rm->W, rm->H);
   }
   hipError_t e = launch_raycast_plain(scene->dev, *rm, d_raybuf, d_line_clip,
-                                      ctx->tuning_block(t, rm->map_line_count), t.run_cache,
+                                      ctx->tuning_block(t, rm->map_line_count, rm->rs), t.run_cache,
                                       (hipStream_t)stream);
   return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "raycast launch");
 }
@@ -223,7 +224,8 @@ rlevox_status rlevox_raycast_counters(rlevox_ctx* ctx, const rlevox_scene* scene
   hipError_t e = launch_fill_u32((uint32_t*)d, 0, sizeof(rlevox_counters) / 4, nullptr);
   if (e == hipSuccess)
     e = launch_raycast_plain(scene->dev, *rm, d_raybuf, d_line_clip,
-                             ctx->tuning_block(ctx->tuning_for(scene, rm->W, rm->H), rm->map_line_count),
+                             ctx->tuning_block(ctx->tuning_for(scene, rm->W, rm->H), rm->map_line_count,
+                                              rm->rs),
                              0, nullptr, d);
   if (e == hipSuccess) e = hipMemcpy(out, d, sizeof(rlevox_counters), hipMemcpyDeviceToHost);
   return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "raycast_counters");
@@ -283,7 +285,7 @@ rlevox_status rlevox_render(rlevox_ctx* ctx, const rlevox_scene* scene, const rl
   int32_t* clip = (int32_t*)ctx->clip.ptr;
   const rlevox_tuning t = ctx->tuning_for(scene, W, H);
   hipError_t e = launch_raycast_plain(scene->dev, rm, raybuf, clip,
-                                      ctx->tuning_block(t, rm.map_line_count), t.run_cache, s);
+                                      ctx->tuning_block(t, rm.map_line_count, rm.rs), t.run_cache, s);
   if (e != hipSuccess) return hip_fail(e, "raycast launch");
   uint16_t* cmp = tb ? (uint16_t*)ctx->ctmp.ptr : d_colour;
   e = launch_compose(rm, raybuf, clip, cmp, d_depth, s);

@@ -99,6 +99,7 @@ hipError_t launch_copy_u128(void* dst, const void* src, uint64_t n16, hipStream_
 struct rlevox_ctx {
   int device = 0;
   int num_cus = 0;
+  size_t max_lds = 65536;  // LDS bytes one workgroup may take
   char arch[64] = {0};
   std::mutex mu;  // guards the scratch buffers and tuning
   rlevox::Scratch raybuf, clip, ctmp, counts;
@@ -129,14 +130,18 @@ struct rlevox_ctx {
   // flythrough, up to 6600 lines: 8% faster with one).
   static constexpr int kTwoWaveLinesPerCU = 14;
   // 1 selects the wavefront-per-line kernel, 2 its two-wavefront form;
-  // otherwise lanes per block (16 or more).
-  int tuning_block(const rlevox_tuning& t, int lines) const {
+  // otherwise lanes per block (16 or more), halved until the block's
+  // per-lane bitmaps (ceil(rs/32) words each) fit the workgroup's LDS.
+  int tuning_block(const rlevox_tuning& t, int lines, int rs) const {
     if (t.lines_per_wave == 1) {
       if (t.waves_per_block == 0)
         return lines <= kTwoWaveLinesPerCU * (num_cus > 0 ? num_cus : 256) ? 2 : 1;
       return t.waves_per_block >= 2 ? 2 : 1;
     }
-    return t.lines_per_wave * t.waves_per_block;
+    int block = t.lines_per_wave * t.waves_per_block;
+    const size_t lane_bytes = (size_t)rlevox::bitmap_words(rs) * sizeof(uint32_t);
+    while (block > 16 && (size_t)block * lane_bytes > max_lds) block >>= 1;
+    return block;
   }
   void release_all() {
     raybuf.release();

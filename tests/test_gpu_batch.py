@@ -95,13 +95,19 @@ def test_batch_compose_only_and_view_order(ctx, scenes):
     ctx.set_tuning()
 
 
-def test_batch_graph_replays(ctx, scenes):
+def test_batch_graph_replays(ctx, scenes, oracle):
     """The captured graph is replayed on three different batches, with eager
-    renders in between; every replay is checked."""
+    renders in between; every replay is checked against plain renders, and
+    its first two views against the oracle."""
     hosts, devs = scenes
     for rep, n in enumerate((16, 5, 30)):
         views = make_views(ctx, hosts, devs, n, seed=10 + rep)
         ctx.render_batch([v["view"] for v in views], graph=True)
+        for v in views[:2]:
+            W, H = v["W"], v["H"]
+            ref = oracle.render(hosts[v["si"]].mips, v["pose"], W, H)
+            assert np.array_equal(v["dc"].download(np.uint16, W * H).reshape(H, W), ref["colour"])
+            assert np.array_equal(v["dd"].download(np.int32, W * H).reshape(H, W), ref["depth"])
         check_views(ctx, devs, views, tag="graph replay %d" % rep)
         free_views(views)
 

@@ -1,7 +1,10 @@
 """The 16384x16384x1024 terrain at full size (GPU): a seeded terrain with
 mips 0-7, whose level 0 needs more than 2^32 slab words (so the encoder
 widens its offsets), rendered at 3840x2160 along grazing poses (|pitch|
-0.6-5 degrees) and compared bit for bit with the oracle."""
+0.6-5 degrees) and compared bit for bit with the oracle; and served next to
+a 4096x4096 terrain in a request mix of 640x480, 1920x1080 and 3840x2160
+views, every served framebuffer against a plain render and a sample
+(every 3840x2160 request on the large terrain) against the oracle."""
 import os
 
 import numpy as np
@@ -38,33 +41,81 @@ def test_c3_scene_layout(c3):
     assert rlevox.validate(c3.mips)[0] == 0
 
 
-def test_c3_grazing_4k_parity(ctx, oracle, c3):
-    from test_gpu_parity import assert_same, gpu_stages
+@pytest.fixture(scope="module")
+def c3_dev(ctx, c3):
     dev = ctx.upload(c3.mips)
+    yield dev
+    dev.free()
+
+
+def test_c3_grazing_4k_parity(ctx, oracle, c3, c3_dev):
+    from test_gpu_parity import assert_same, gpu_stages
+    dev = c3_dev
+    assert dev.device_bytes > 15e9
+    for pose in workloads.grazing(c3.mips[0], n=2, seed=11):
+        rm = rlevox.raymap(pose, 3840, 2160)
+        ref_rb, ref_clip, _ = oracle.raycast(c3.mips, oracle.raymap(pose, 3840, 2160))
+        ref_col, ref_dep = oracle.compose(rm, ref_rb, ref_clip)
+        ref_fin = oracle.gapfill(ref_col, ref_dep)
+        got = gpu_stages(ctx, dev, rm)
+        n = max(rm.map_line_count, 1)
+        assert_same("line clip", got["clip"][:n], ref_clip[:n])
+        assert_same("ray buffer", got["raybuf"], ref_rb)
+        assert_same("compose colour", got["compose"], ref_col)
+        assert_same("depth", got["depth"], ref_dep)
+        assert_same("gap-filled colour", got["colour"], ref_fin)
+        assert np.mean(ref_dep < 0x7FFFFFFF) > 0.05  # the views see terrain
+        # both kernel shapes, forced, give the same frame (the automatic
+        # choice above takes one wavefront per line at this line count)
+        for wpl in (1, 2):
+            ctx.set_tuning(waves_per_block=wpl)
+            try:
+                other = gpu_stages(ctx, dev, rm)
+            finally:
+                ctx.set_tuning()
+            for k in ("clip", "raybuf", "compose", "depth", "colour"):
+                assert_same("%s, %d wavefront(s) per line" % (k, wpl), other[k], got[k])
+
+
+def test_c5_served_mix_with_the_large_terrain(ctx, oracle, c3, c3_dev):
+    """Requests over the 16384^2 terrain and a 4096^2 one, resolutions drawn
+    0.5 / 0.35 / 0.15 from 640x480 / 1920x1080 / 3840x2160 (three 3840x2160
+    requests on the large terrain always among them), submitted in bursts to
+    a server with three lanes."""
+    from test_gpu_parity import assert_same
+    c2 = workloads.gen("c2", seed=1)
+    c2_dev = ctx.upload(c2.mips)
+    hosts, devs = [c3, c2], [c3_dev, c2_dev]
+    rng = np.random.default_rng(5)
+    reqs = [(0, 3840, 2160), (1, 1920, 1080), (0, 3840, 2160), (0, 640, 480), (0, 3840, 2160)]
+    for _ in range(11):
+        u = rng.random()
+        W, H = (640, 480) if u < 0.5 else ((1920, 1080) if u < 0.85 else (3840, 2160))
+        reqs.append((int(rng.integers(2)), W, H))
+    poses = [workloads.random_views(hosts[si].mips[0], 1, seed=100 + k)[0]
+             for k, (si, _, _) in enumerate(reqs)]
+    srv = rlevox.Server(ctx, memory_budget=16 << 30, max_batch_views=6, streams=3)
     try:
-        assert dev.device_bytes > 15e9
-        for pose in workloads.grazing(c3.mips[0], n=2, seed=11):
-            rm = rlevox.raymap(pose, 3840, 2160)
-            ref_rb, ref_clip, _ = oracle.raycast(c3.mips, oracle.raymap(pose, 3840, 2160))
-            ref_col, ref_dep = oracle.compose(rm, ref_rb, ref_clip)
-            ref_fin = oracle.gapfill(ref_col, ref_dep)
-            got = gpu_stages(ctx, dev, rm)
-            n = max(rm.map_line_count, 1)
-            assert_same("line clip", got["clip"][:n], ref_clip[:n])
-            assert_same("ray buffer", got["raybuf"], ref_rb)
-            assert_same("compose colour", got["compose"], ref_col)
-            assert_same("depth", got["depth"], ref_dep)
-            assert_same("gap-filled colour", got["colour"], ref_fin)
-            assert np.mean(ref_dep < 0x7FFFFFFF) > 0.05  # the views see terrain
-            # both kernel shapes, forced, give the same frame (the automatic
-            # choice above takes one wavefront per line at this line count)
-            for wpl in (1, 2):
-                ctx.set_tuning(waves_per_block=wpl)
-                try:
-                    other = gpu_stages(ctx, dev, rm)
-                finally:
-                    ctx.set_tuning()
-                for k in ("clip", "raybuf", "compose", "depth", "colour"):
-                    assert_same("%s, %d wavefront(s) per line" % (k, wpl), other[k], got[k])
+        tickets = [srv.submit(devs[si], p, W, H) for (si, W, H), p in zip(reqs, poses)]
+        for k, ((si, W, H), p, t) in enumerate(zip(reqs, poses, tickets)):
+            col, dep, lat = srv.wait(t, timeout_ms=120000)
+            tag = "request %d (%dx%d on the %s terrain)" % (k, W, H, "16384^2" if si == 0 else
+                                                           "4096^2")
+            print("c5 " + tag, flush=True)
+            if (si == 0 and W == 3840) or k in (1, 3):
+                ref = oracle.render(hosts[si].mips, p, W, H)
+                assert_same(tag + " colour", col, ref["colour"])
+                assert_same(tag + " depth", dep, ref["depth"])
+            d_col, d_dep = ctx.alloc(W * H * 2), ctx.alloc(W * H * 4)
+            ctx.render(devs[si], p, W, H, d_col, d_dep)
+            assert_same(tag + " colour vs plain", col,
+                        d_col.download(np.uint16, W * H).reshape(H, W))
+            assert_same(tag + " depth vs plain", dep,
+                        d_dep.download(np.int32, W * H).reshape(H, W))
+            d_col.free()
+            d_dep.free()
+        assert srv.stats()["completed"] == len(reqs)
     finally:
-        dev.free()
+        srv.close()
+        c2_dev.free()
+        c2.close()

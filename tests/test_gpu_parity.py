@@ -278,3 +278,35 @@ def test_many_elements_per_column(ctx, oracle, layered_scene):
             dc.free()
             dd.free()
         dev.free()
+
+
+def test_wide_views_fit_the_workgroup_lds(ctx, oracle, tiny_scene, tiny_dev):
+    """7680x4320: a lane-per-line block keeps ceil(7680/32) bitmap words per
+    lane in LDS, more than one workgroup may hold at 256 lanes; the library
+    shrinks such blocks, so every tuning (and the autotuner over them)
+    renders the oracle's frame."""
+    W, H = 7680, 4320
+    pose = workloads.random_views(tiny_scene.mips[0], 1, seed=77, alt=(4.0, 30.0))[0]
+    ref = oracle.render(tiny_scene.mips, pose, W, H)
+    d_col, d_dep = ctx.alloc(W * H * 2), ctx.alloc(W * H * 4)
+    try:
+        for lpw, wpb in [(1, 0), (64, 4), (64, 2), (32, 2), (16, 4)]:
+            ctx.set_tuning(lines_per_wave=lpw, waves_per_block=wpb)
+            d_col.fill_u32(0)
+            ctx.render(tiny_dev, pose, W, H, d_col, d_dep)
+            tag = " [lines/wave %d, waves/block %d]" % (lpw, wpb)
+            assert_same("colour" + tag, d_col.download(np.uint16, W * H).reshape(H, W),
+                        ref["colour"])
+            assert_same("depth" + tag, d_dep.download(np.int32, W * H).reshape(H, W),
+                        ref["depth"])
+        ctx.set_tuning()
+        best, ms = ctx.autotune([rlevox.make_view(tiny_dev, pose, W, H, d_col, d_dep)])
+        assert ms > 0
+        d_col.fill_u32(0)
+        ctx.render(tiny_dev, pose, W, H, d_col, d_dep)
+        assert_same("colour [autotuned]", d_col.download(np.uint16, W * H).reshape(H, W),
+                    ref["colour"])
+    finally:
+        ctx.set_tuning()
+        d_col.free()
+        d_dep.free()

@@ -49,8 +49,9 @@ def requests(c1_scene, tiny_scene, n, seed):
     return out
 
 
-def test_served_results_equal_plain_renders(ctx, scenes, c1_scene, tiny_scene):
+def test_served_results_equal_plain_renders(ctx, scenes, c1_scene, tiny_scene, oracle):
     reqs = requests(c1_scene, tiny_scene, 40, seed=21)
+    hosts = [c1_scene, tiny_scene]
     want = [plain_render(ctx, scenes[si], p, W, H) for si, p, W, H in reqs]
     srv = rlevox.Server(ctx, memory_budget=2 << 30, max_batch_views=8, streams=3)
     try:
@@ -64,6 +65,11 @@ def test_served_results_equal_plain_renders(ctx, scenes, c1_scene, tiny_scene):
             assert np.array_equal(col, want[k][0]), "request %d colour" % k
             assert np.array_equal(dep, want[k][1]), "request %d depth" % k
             assert lat > 0
+            if k % 8 == 0:  # a sample straight against the oracle
+                si, p, W, H = reqs[k]
+                ref = oracle.render(hosts[si].mips, p, W, H)
+                assert np.array_equal(col, ref["colour"]), "request %d colour vs oracle" % k
+                assert np.array_equal(dep, ref["depth"]), "request %d depth vs oracle" % k
         st = srv.stats()
         assert st["completed"] == len(reqs) and st["submitted"] == len(reqs)
         assert st["batches"] >= 5  # at most 8 views per batch
@@ -164,7 +170,7 @@ def test_destroy_with_requests_in_flight(ctx, scenes, c1_scene):
     srv.close()  # drains the batches on the device, drops the queue
 
 
-def test_autotune_keeps_results(ctx, scenes, c1_scene):
+def test_autotune_keeps_results(ctx, scenes, c1_scene, oracle):
     poses = workloads.random_views(c1_scene.mips[0], 3, seed=12)
     bufs = [(ctx.alloc(1280 * 720 * 2), ctx.alloc(1280 * 720 * 4)) for _ in poses]
     views = [rlevox.make_view(scenes[0], p, 1280, 720, c, d) for p, (c, d) in zip(poses, bufs)]
@@ -175,6 +181,8 @@ def test_autotune_keeps_results(ctx, scenes, c1_scene):
     for p, w in zip(poses, want):
         got = plain_render(ctx, scenes[0], p, 1280, 720)
         assert np.array_equal(got[0], w[0]) and np.array_equal(got[1], w[1])
+        ref = oracle.render(c1_scene.mips, p, 1280, 720)  # the tuned render vs the oracle
+        assert np.array_equal(got[0], ref["colour"]) and np.array_equal(got[1], ref["depth"])
     ctx.set_tuning()
     for c, d in bufs:
         c.free()
