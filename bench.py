@@ -3,7 +3,7 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch]
 
 Workload (plain): a 4096x4096x1024 seeded terrain (8-octave value noise,
-~5 runs per column, mips 0-5) resident in HBM, rendered at 1920x1080 along a
+5.6-5.9 runs per column, mips 0-5) resident in HBM, rendered at 1920x1080 along a
 seeded 100-frame flythrough.  One step = one frame: ray map (host) + raycast
 + compose + gap-fill on the device; inputs are resident before timing starts.
 `value` is Mrays/s (one ray = one screen pixel) over the K timed frames.
@@ -11,7 +11,9 @@ seeded 100-frame flythrough.  One step = one frame: ray map (host) + raycast
 The roofline object is for the dominant kernel (the raycast): its algorithmic
 bytes per launch come from the exact work counts of every timed frame
 (8 B per pointer-map entry read, 2 B per extra RLE element, 2 B per colour,
-4 B per texel cleared and per texel written), divided by its mean duration
+4 B per texel cleared and per texel written) of the walk the kernels make,
+in which a line ends as soon as no later step can draw (the same counts over
+the reference's walk to the far distance are reported beside them), divided by its mean duration
 from HIP events recorded on the launch stream around each launch.  The
 events are recorded in a second timed pass over the same K frames: a timed
 event costs about 3 us of device time per record, which would otherwise
@@ -94,9 +96,16 @@ def main():
     ap.add_argument("--serve-batch", type=int, default=48)
     ap.add_argument("--serve-streams", type=int, default=3)
     ap.add_argument("--serve-budget-gb", type=int, default=48)
-    ap.add_argument("--serve-c3", type=int, default=0, metavar="K",
+    ap.add_argument("--serve-c3", type=int, default=3, metavar="K",
                     help="make K of the 12 resident scenes device copies of the "
-                         "16384^2 x 1024 terrain (~31 GB each) instead of 4096^2 ones")
+                         "16384^2 x 1024 terrain (~33 GB each) instead of 4096^2 ones "
+                         "(default 3: ~105 GB resident)")
+    ap.add_argument("--serve-large-lanes", type=int, default=1,
+                    help="server lanes reserved for 3840x2160 requests (0: any lane takes any)")
+    ap.add_argument("--serve-large-waves", type=int, default=8,
+                    help="persistent raycast wavefronts per CU on the large lanes")
+    ap.add_argument("--serve-batch-mpix", type=float, default=8.0,
+                    help="pixel cap of one served batch, in millions (0: none)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--waves-per-line", type=int, choices=(0, 1, 2), default=0,
                     help="wavefronts walking each ray line (2: DDA and drawing overlap; "
@@ -145,8 +154,11 @@ def main():
     d_col = ctx.alloc(W * H * 2)
     d_dep = ctx.alloc(W * H * 4)
 
-    # exact work counts per pose (counting kernel, untimed)
-    counts = [ctx.raycast_counters(scene, rm, d_rb, d_clip) for rm in rms]
+    # exact work counts per pose (counting kernel, untimed): the walk as the
+    # kernels make it (a line ends once nothing more can draw), and the
+    # reference's walk of every line to its far distance
+    counts = [ctx.raycast_counters(scene, rm, d_rb, d_clip, walk=True) for rm in rms]
+    counts_ref = [ctx.raycast_counters(scene, rm, d_rb, d_clip) for rm in rms]
     log("work counts done")
 
     def frame(i, ev=None):
@@ -184,6 +196,9 @@ def main():
     mrays = K * W * H / wall / 1e6
     bytes_per = [raycast_bytes(counts[i % len(poses)]) for i in range(K)]
     mean_bytes = float(np.mean(bytes_per))
+    mean_bytes_ref = float(np.mean([raycast_bytes(counts_ref[i % len(poses)]) for i in range(K)]))
+    steps_frac = (sum(counts[i % len(poses)]["dda_steps"] for i in range(K))
+                  / max(1, sum(counts_ref[i % len(poses)]["dda_steps"] for i in range(K))))
     mean_ray_ms = float(np.mean(ray_ms))
     achieved = mean_bytes / (mean_ray_ms * 1e-3) / 1e9
     log("plain: %.2f ms/frame, %.1f Mrays/s; raycast kernel mean %.3f ms (p50 %.3f, p99 %.3f), "
@@ -214,6 +229,11 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": measured_traffic(kernel), "kernel": kernel,
                      "bytes_per_launch": round(mean_bytes),
+                     # the same counted over the reference's walk (every line to
+                     # its far distance); the kernels' walk takes this share of
+                     # its DDA steps
+                     "bytes_reference_walk": round(mean_bytes_ref),
+                     "dda_steps_vs_reference": round(steps_frac, 4),
                      "traffic_source": traffic_profile(kernel) + " (rocprofv3 FETCH_SIZE + "
                                        "WRITE_SIZE, copy-kernel calibrated)",
                      # diagnostics from the same profiles: L2 hit rate (TCC_HIT /
@@ -322,7 +342,8 @@ def bench_c3(ctx, rlevox, workloads, frames=8, reps=3):
     lines = max(r.map_line_count for r in rms)
     d_rb, d_clip = ctx.alloc(lines * W * 4), ctx.alloc(lines * 8)
     d_c0, d_col, d_dep = ctx.alloc(W * H * 2), ctx.alloc(W * H * 2), ctx.alloc(W * H * 4)
-    counts = [ctx.raycast_counters(scene, rm, d_rb, d_clip) for rm in rms]
+    counts = [ctx.raycast_counters(scene, rm, d_rb, d_clip, walk=True) for rm in rms]
+    counts_ref = [ctx.raycast_counters(scene, rm, d_rb, d_clip) for rm in rms]
     stream = ctx.stream()
     ev = [(ctx.event(), ctx.event()) for _ in range(frames * reps)]
     for rm in rms:  # warm-up
@@ -348,6 +369,8 @@ def bench_c3(ctx, rlevox, workloads, frames=8, reps=3):
            "ms_per_frame": round(wall * 1e3 / (frames * reps), 3),
            "raycast_ms_mean": round(ray_ms, 3),
            "raycast_bytes_per_launch": round(mean_bytes),
+           "raycast_bytes_reference_walk": round(float(np.mean([raycast_bytes(c)
+                                                                for c in counts_ref]))),
            "raycast_gbs": round(mean_bytes / (ray_ms * 1e-3) / 1e9, 2)}
     # measured HBM traffic and L2 hit rate of the same workload (rocprofv3
     # FETCH_SIZE / WRITE_SIZE / TCC passes, copy-kernel calibrated)
@@ -429,13 +452,22 @@ def bench_served(ctx, rlevox, workloads, scenes_h, scenes, args):
     out = {"requests": n, "resident_scenes": len(resident), "rays": rays,
            "resident_gb": round(sum(s.device_bytes for s in resident) / 1e9, 2),
            "resident_16384_copies": args.serve_c3,
+           "resident_mix": "%d device copies of the 4096^2x1024 terrains (seeds 1-4), %d of "
+                           "the 16384^2x1024 terrain (seed 1)" % (len(resident) - args.serve_c3,
+                                                                  args.serve_c3),
            "mix": "640x480:0.5 1920x1080:0.35 3840x2160:0.15",
+           "server": {"streams": args.serve_streams, "max_batch_views": args.serve_batch,
+                      "large_lanes": args.serve_large_lanes,
+                      "large_waves": args.serve_large_waves,
+                      "batch_mpix": args.serve_batch_mpix},
            "plain_mrays_s": round(rays / plain_s / 1e6, 2)}
 
     def run(rate):
         srv = rlevox.Server(ctx, memory_budget=args.serve_budget_gb << 30,
                             max_batch_views=args.serve_batch, max_queue=1 << 20,
-                            streams=args.serve_streams)
+                            streams=args.serve_streams, large_lanes=args.serve_large_lanes,
+                            large_pixels=3840 * 2160, large_waves=args.serve_large_waves,
+                            batch_pixels=int(args.serve_batch_mpix * 1e6))
         tickets = [None] * n
         t_start = time.perf_counter()
         if rate is None:

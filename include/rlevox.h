@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define RLEVOX_ABI_VERSION 2
+#define RLEVOX_ABI_VERSION 3
 #define RLEVOX_MAX_MIPS 16
 /* Cleared ray-buffer texel (Cuda_Render.h:257). Its byte 2 (0xff) marks sky. */
 #define RLEVOX_CLEAR_TEXEL 0x00ff8844u
@@ -137,6 +137,13 @@ typedef struct {
   uint64_t spans;       /* spans drawn */
   uint64_t col_loads;   /* colours read = texels drawn (2 B) */
   uint64_t cleared;     /* texels cleared (4 B) */
+  /* The same for the walk as the kernels make it: a line's walk ends as soon
+   * as no later step can draw (the fields above count the reference's walk
+   * to the far distance; spans, colours and cleared texels are equal). */
+  uint64_t walk_steps;
+  uint64_t walk_ptr_loads;
+  uint64_t walk_rle_loads;
+  uint64_t walk_elems;
 } rlevox_counters;
 
 /* Seeded synthetic terrain: value-noise heightfield, layered caves, per-voxel
@@ -185,7 +192,12 @@ typedef struct {
                               40 with two wavefronts per line, 56 with one):
                               deeper holds more of a column's runs and
                               colours in LDS, shallower loads less per
-                              column.  The persistent batch kernel uses 40. */
+                              column */
+  int32_t batch_run_cache; /* the same for the persistent batch kernel (0 = 40) */
+  int32_t batch_par;       /* element replay of the persistent batch kernel:
+                              0 = chosen by the scenes' runs per column
+                              (lane-parallel above 7), 1 = element by
+                              element, 2 = lane-parallel preparation */
 } rlevox_tuning;
 
 typedef struct {
@@ -193,8 +205,19 @@ typedef struct {
   int32_t max_batch_views; /* views packed into one launch */
   int32_t max_queue;       /* pending requests before submit says QUEUE_FULL */
   int32_t streams;         /* batches in flight at once (1..8) */
-  int32_t reserved;
+  int32_t large_lanes;     /* 0: every lane takes any request.  k (1..streams-1):
+                              the first k lanes take only large requests and
+                              the others only the rest, so a small request
+                              never shares a batch with a large one */
   rlevox_render_params params;
+  uint64_t batch_pixels;   /* a batch stops taking requests once their W*H
+                              sum reaches this (it always takes one; 0 = no
+                              limit): bounds how long a batch holds a lane */
+  int64_t large_pixels;    /* W*H from which a request is large (large_lanes) */
+  int32_t large_waves;     /* persistent raycast wavefronts per CU in a large
+                              lane's batches (0 = the batch default; fewer
+                              leave room on the GPU to the other lanes) */
+  int32_t reserved;
 } rlevox_server_config;
 
 typedef struct {
@@ -328,6 +351,19 @@ rlevox_status rlevox_get_tuning_for(rlevox_ctx* ctx, const rlevox_scene* scene, 
 rlevox_status rlevox_autotune(rlevox_ctx* ctx, const rlevox_view* views, int32_t nviews,
                               const rlevox_render_params* params, rlevox_tuning* best,
                               double* best_ms);
+/* Autotuner of the batched path (no counterpart in the reference, which
+ * fixes its launch shape at compile time: core.h:10, Cuda_Main.cu:195-202,
+ * 226): times rlevox_render_batch on the given views as one batch for every
+ * combination of persistent wavefronts per CU (8, 12, 16, 24), view order
+ * (0, 1), batch run-cache depth (24, 40, 56) and element replay (1, 2), and
+ * makes the fastest the context's batch settings (persistent_waves, order,
+ * batch_run_cache, batch_par of its tuning), which rlevox_render_batch,
+ * rlevox_render_batch_graph and servers created on the context use.  The
+ * views' framebuffers are rendered into.  *best receives the context's
+ * tuning after the change, *best_ms the winner's batch time. */
+rlevox_status rlevox_autotune_batch(rlevox_ctx* ctx, const rlevox_view* views, int32_t nviews,
+                                    const rlevox_render_params* params, rlevox_tuning* best,
+                                    double* best_ms);
 
 /* ------------------------------------------------------ device: serving */
 

@@ -391,6 +391,21 @@ int orc_raymap(const rlevox_pose* pose, int W, int H, int mip_distance, int far_
   return 0;
 }
 
+// Hooks a host check can define before including this file to observe the
+// walk of each line (tests/native/walk_end_check.cpp); empty otherwise.
+#ifndef ORC_HOOK_LINE_BEGIN
+#define ORC_HOOK_LINE_BEGIN
+#endif
+#ifndef ORC_HOOK_STEP
+#define ORC_HOOK_STEP
+#endif
+#ifndef ORC_HOOK_ACT
+#define ORC_HOOK_ACT
+#endif
+#ifndef ORC_HOOK_LINE_END
+#define ORC_HOOK_LINE_END
+#endif
+
 // ------------------------------------------------------------ render line
 struct OrcScene {
   const rlevox_mip* mips;
@@ -559,6 +574,7 @@ static void orc_render_line(int x, const OrcScene& sc, const rlevox_raymap& rm, 
 
   auto bit = [&](int y) -> bool { return (y_cache[y >> 5] >> (y & 31)) & 1u; };
 
+  ORC_HOOK_LINE_BEGIN
   for (;;) {
     // Mip switch (Cuda_Render.h:343-367)
     while (z > (float)mapswitch || y_map_switch > 512.0f) {
@@ -580,6 +596,7 @@ static void orc_render_line(int x, const OrcScene& sc, const rlevox_raymap& rm, 
       mapswitch *= 2;
       dz *= 2;
     }
+    ORC_HOOK_STEP
     z += dz;
     if (z > (float)z_far) break;
     if (y_clip_min >= y_clip_max) break;
@@ -691,6 +708,7 @@ static void orc_render_line(int x, const OrcScene& sc, const rlevox_raymap& rm, 
 #endif
       if (scr_y1 >= y_clip_max) continue;
 
+      ORC_HOOK_ACT
       // Floating horizon (Cuda_Render.h:563-580)
       if (scr_y2 >= y_clip_max) {
         scr_y2 = y_clip_max;
@@ -745,6 +763,7 @@ static void orc_render_line(int x, const OrcScene& sc, const rlevox_raymap& rm, 
       }
     }
   }
+  ORC_HOOK_LINE_END
   if (ctr) {
     ctr->lines += 1;
     ctr->dda_steps += c_steps;

@@ -148,11 +148,28 @@ rlevox_status rlevox_scene_upload(rlevox_ctx* ctx, const rlevox_mip* mips, int32
     s->sy[i] = m[i].sy;
     s->bytes += map_bytes + slab_bytes;
   }
+  // Device copy of the descriptor; behind it, scratch for the highest run
+  // top of each level (the early end of a line's walk needs the top of the
+  // levels still to come).
   void* dd = nullptr;
-  hipError_t e = dev_malloc(&dd, sizeof(SceneDev));
+  hipError_t e = dev_malloc(&dd, sizeof(SceneDev) + sizeof(int32_t) * RLEVOX_MAX_MIPS);
   if (e == hipSuccess) {
     s->allocs.push_back(dd);
-    e = hipMemcpy(dd, &s->dev, sizeof(SceneDev), hipMemcpyHostToDevice);
+    int32_t* d_top = (int32_t*)((char*)dd + sizeof(SceneDev));
+    e = launch_fill_u32((uint32_t*)d_top, (uint32_t)INT32_MAX, RLEVOX_MAX_MIPS, nullptr);
+    for (int i = 0; i < nmips && e == hipSuccess; i++)
+      e = launch_column_tops(s->dev.mip[i], i, d_top + i, nullptr);
+    int32_t top[RLEVOX_MAX_MIPS];
+    if (e == hipSuccess) e = hipMemcpy(top, d_top, sizeof(top), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) {
+      int32_t t = m[0].sy;  // a level without drawn runs: nothing above the column bottom
+      for (int i = nmips - 1; i >= 0; i--) {
+        t = std::min(t, top[i]);
+        s->dev.mip[i].top = t;
+      }
+      s->dev.sy = m[0].sy;
+      e = hipMemcpy(dd, &s->dev, sizeof(SceneDev), hipMemcpyHostToDevice);
+    }
   }
   if (e != hipSuccess) {
     rlevox_scene_free(s);

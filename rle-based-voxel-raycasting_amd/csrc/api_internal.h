@@ -30,7 +30,10 @@ inline bool tuning_valid(const rlevox_tuning& t) {
          t.waves_per_block >= (t.lines_per_wave == 1 ? 0 : 1) && t.waves_per_block <= 4 &&
          (t.persistent_waves == 0 || (t.persistent_waves >= 4 && t.persistent_waves <= 32)) &&
          (t.order == 0 || t.order == 1) &&
-         (t.run_cache == 0 || t.run_cache == 24 || t.run_cache == 40 || t.run_cache == 56);
+         (t.run_cache == 0 || t.run_cache == 24 || t.run_cache == 40 || t.run_cache == 56) &&
+         (t.batch_run_cache == 0 || t.batch_run_cache == 24 || t.batch_run_cache == 40 ||
+          t.batch_run_cache == 56) &&
+         (t.batch_par >= 0 && t.batch_par <= 2);
 }
 
 inline int post_flags(const rlevox_render_params* p) {
@@ -81,8 +84,10 @@ struct BatchSlot {
   int nviews = 0;
   int waves = kWavesPerCU;        // persistent wavefronts per CU of this batch
   bool par_elems = false;         // raycast with the lane-parallel element replay
+  int run_cache = 40;             // LDS run-cache words of the raycast
   int graph_waves = 0;            // the values the captured graph was made with
   bool graph_par = false;
+  int graph_rc = 0;
   uint64_t rays = 0;
   rlevox_status init(rlevox_ctx* c);
   rlevox_status prepare(const rlevox_view* views, int n, const rlevox_render_params* params,
@@ -104,7 +109,7 @@ struct rlevox_ctx {
   std::mutex mu;  // guards the scratch buffers and tuning
   rlevox::Scratch raybuf, clip, ctmp, counts;
   rlevox::BatchSlot batch;
-  rlevox_tuning tuning{1, 0, 0, 1, 0};
+  rlevox_tuning tuning{1, 0, 0, 1, 0, 0, 0};
   // Autotuned settings per (scene, W, H); others use `tuning`.
   struct TuneKey {
     const rlevox_scene* scene;

@@ -33,8 +33,9 @@ struct BatchHeader {
 };
 
 // par_elems: the raycast replays a column's elements with the lane-parallel
-// preparation (the plain kernels' replay) instead of one by one.
+// preparation (the plain kernels' replay) instead of one by one; run_cache:
+// LDS run-cache words per candidate column (24, 40 or 56).
 hipError_t launch_batch(BatchHeader* d_hdr, const ViewDesc* d_views, int raycast_grid, int post_grid,
-                        bool par_elems, hipStream_t s);
+                        bool par_elems, int run_cache, hipStream_t s);
 
 }  // namespace rlevox

@@ -42,13 +42,15 @@ __device__ inline int find_view(const ViewDesc* __restrict__ v, int n, int64_t g
 // visits only those that can act, as the plain kernels do; it spills more
 // and pays only where columns hold many runs (served mix with three 16384^2
 // scenes resident: 4165 -> 4769 Mrays/s; 64 views over 4096^2 terrains:
-// 10.36 -> 10.86 ms), so a batch takes it by its scenes' runs per column.
-template <bool PAR>
+// 10.36 -> 10.86 ms), so a batch takes it by its scenes' runs per column
+// unless the tuning says otherwise.  RC: LDS run-cache words per column.
+template <bool PAR, int RC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_raycast_batch(BatchHeader* __restrict__ h,
                                                       const ViewDesc* __restrict__ views) {
+  static_assert(kDdaLds * 4 <= 64 * RC * 2, "DDA scratch must fit in the run cache");
   __shared__ uint32_t bits[(RLEVOX_MAX_RS + 31) / 32];
-  __shared__ uint4 rcache[64 * kRunCache / 8];
-  __shared__ int sweep[64];
+  __shared__ uint4 rcache[64 * RC / 8];
+  __shared__ int sweep[64 + kWalkEndWords];
   float* dda = reinterpret_cast<float*>(rcache);  // see kDdaLds
   const int lane = threadIdx.x;
   const int64_t total = h->total_lines;
@@ -63,7 +65,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const ViewDesc& vd = views[v];
     // lines of a view from the last to the first, as the single-view kernels
     const int x = vd.rm.map_line_count - 1 - (int)((int64_t)g - vd.line_begin);
-    render_line_wave<false, false, PAR>(x, *vd.scene, vd.rm,
+    render_line_wave<false, false, PAR, RC>(x, *vd.scene, vd.rm,
                                           vd.raybuf + (size_t)x * (size_t)vd.rm.rs,
                             vd.clip + 2 * x, bits, dda, reinterpret_cast<uint16_t*>(rcache),
                             sweep, nullptr);
@@ -131,13 +133,24 @@ __global__ __launch_bounds__(256) void k_gapfill_batch(const BatchHeader* __rest
   }
 }
 
+template <bool PAR>
+static void launch_raycast_batch(BatchHeader* d_hdr, const ViewDesc* d_views, int grid, int rc,
+                                 hipStream_t s) {
+  if (rc == 24)
+    hipLaunchKernelGGL((k_raycast_batch<PAR, 24>), dim3(grid), dim3(64), 0, s, d_hdr, d_views);
+  else if (rc == 56)
+    hipLaunchKernelGGL((k_raycast_batch<PAR, 56>), dim3(grid), dim3(64), 0, s, d_hdr, d_views);
+  else
+    hipLaunchKernelGGL((k_raycast_batch<PAR, 40>), dim3(grid), dim3(64), 0, s, d_hdr, d_views);
+}
+
 hipError_t launch_batch(BatchHeader* d_hdr, const ViewDesc* d_views, int raycast_grid, int post_grid,
-                        bool par_elems, hipStream_t s) {
+                        bool par_elems, int run_cache, hipStream_t s) {
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, d_hdr);
   if (par_elems)
-    hipLaunchKernelGGL(k_raycast_batch<true>, dim3(raycast_grid), dim3(64), 0, s, d_hdr, d_views);
+    launch_raycast_batch<true>(d_hdr, d_views, raycast_grid, run_cache, s);
   else
-    hipLaunchKernelGGL(k_raycast_batch<false>, dim3(raycast_grid), dim3(64), 0, s, d_hdr, d_views);
+    launch_raycast_batch<false>(d_hdr, d_views, raycast_grid, run_cache, s);
   hipLaunchKernelGGL(k_compose_batch, dim3(post_grid), dim3(256), 0, s, d_hdr, d_views);
   hipLaunchKernelGGL(k_gapfill_batch, dim3(post_grid), dim3(256), 0, s, d_hdr, d_views);
   return hipGetLastError();
@@ -198,6 +211,8 @@ rlevox_status BatchSlot::prepare(const rlevox_view* views, int n, const rlevox_r
   constexpr float kParRuns = 7.0f;
   par_elems = false;
   for (int i = 0; i < n; i++) par_elems = par_elems || views[i].scene->mean_runs > kParRuns;
+  if (tuning.batch_par) par_elems = tuning.batch_par == 2;
+  run_cache = tuning.batch_run_cache ? tuning.batch_run_cache : 40;
   // scratch layout: per view ray buffer, line clip ranges, composed colour
   size_t bytes = 0;
   std::vector<size_t> off_rb(n), off_clip(n), off_ct(n);
@@ -273,10 +288,11 @@ rlevox_status BatchSlot::run(bool use_graph, hipStream_t s) {
   const int ray_grid = cus * waves;
   const int post_grid = cus * 8;
   if (!use_graph) {
-    e = launch_batch(d_hdr, d_views, ray_grid, post_grid, par_elems, s);
+    e = launch_batch(d_hdr, d_views, ray_grid, post_grid, par_elems, run_cache, s);
     return e == hipSuccess ? RLEVOX_OK : hip_fail(e, "batch launch");
   }
-  if (exec && (graph_waves != waves || graph_par != par_elems)) {  // kernels changed: capture again
+  if (exec && (graph_waves != waves || graph_par != par_elems || graph_rc != run_cache)) {
+    // the kernels or their grid changed: capture again
     hipGraphExecDestroy(exec);
     hipGraphDestroy(graph);
     exec = nullptr;
@@ -285,11 +301,12 @@ rlevox_status BatchSlot::run(bool use_graph, hipStream_t s) {
   if (!exec) {
     graph_waves = waves;
     graph_par = par_elems;
+    graph_rc = run_cache;
     hipStream_t cap = nullptr;
     e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      hipError_t le = launch_batch(d_hdr, d_views, ray_grid, post_grid, par_elems, cap);
+      hipError_t le = launch_batch(d_hdr, d_views, ray_grid, post_grid, par_elems, run_cache, cap);
       e = hipStreamEndCapture(cap, &graph);
       if (e == hipSuccess) e = le;
     }

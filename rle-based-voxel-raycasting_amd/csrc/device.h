@@ -13,6 +13,9 @@
 
 #include "../../include/rlevox.h"
 
+#define RLEVOX_HD __host__ __device__
+#include "walk_end.h"
+
 namespace rlevox {
 
 // One mip level in HBM: the pointer map as 8-byte entries, the slab array.
@@ -21,13 +24,13 @@ struct MipDev {
   const uint16_t* slabs;
   int32_t sx, sz;
   int32_t shift;  // column word offset = map[c].x << shift
-  int32_t pad;
+  int32_t top;    // highest run top (smallest world row) of this level and all coarser ones
 };
 
 struct SceneDev {
   MipDev mip[RLEVOX_MAX_MIPS];
   int32_t nmips;
-  int32_t pad;
+  int32_t sy;  // column height in world rows (level 0)
 };
 
 // Ray-buffer line bitmap words for render size rs.
@@ -84,6 +87,32 @@ __device__ inline F3 mat_apply(const float (*m)[4], F3 v) {
 // Cuda_Render.h:14-22, extended); only the counting kernel variant keeps them.
 struct LineCounts {
   uint64_t steps = 0, ptr = 0, rle = 0, elems = 0, spans = 0, col = 0, cleared = 0;
+  // the same at the point where the kernels end the walk
+  bool ended = false;
+  uint64_t wsteps = 0, wptr = 0, wrle = 0, welems = 0;
+  __device__ void end_walk() {
+    if (ended) return;
+    ended = true;
+    wsteps = steps;
+    wptr = ptr;
+    wrle = rle;
+    welems = elems;
+  }
+  __device__ void flush(unsigned long long* counts) {
+    end_walk();
+    atomicAdd(counts + 0, 1ull);
+    atomicAdd(counts + 1, (unsigned long long)steps);
+    atomicAdd(counts + 2, (unsigned long long)ptr);
+    atomicAdd(counts + 3, (unsigned long long)rle);
+    atomicAdd(counts + 4, (unsigned long long)elems);
+    atomicAdd(counts + 5, (unsigned long long)spans);
+    atomicAdd(counts + 6, (unsigned long long)col);
+    atomicAdd(counts + 7, (unsigned long long)cleared);
+    atomicAdd(counts + 8, (unsigned long long)wsteps);
+    atomicAdd(counts + 9, (unsigned long long)wptr);
+    atomicAdd(counts + 10, (unsigned long long)wrle);
+    atomicAdd(counts + 11, (unsigned long long)welems);
+  }
 };
 
 // Raycasts ray line x of one view into its ray-buffer row and records the
@@ -224,6 +253,9 @@ __device__ inline void render_line(int x, const SceneDev& sc, const rlevox_rayma
   float dz = 1, z = 0;
   float yswitch = vpy;
   const float mountain = vpy;
+  WalkEnd wend;
+  wend.init(cos_x, sin_x, hmul, ray_x, ray_z, vertical, mountain, half);
+  int since_end_test = 0;
 
   for (;;) {
     // ---- distance mip switch (Cuda_Render.h:343-367)
@@ -245,6 +277,14 @@ __device__ inline void render_line(int x, const SceneDev& sc, const rlevox_rayma
       gd1 *= 2;
       mapswitch *= 2;
       dz *= 2;
+    }
+    // ---- early end of the walk (WalkEnd), tested every 64 steps
+    if (++since_end_test == 64) {
+      since_end_test = 0;
+      if (ymin < ymax && !cnt.ended && wend.done(dnow, ymin, sc.mip[mip].top, sc.sy)) {
+        if (!COUNT) break;
+        cnt.end_walk();  // the counting variant walks on to count the reference's work
+      }
     }
     z += dz;
     if (z > zfar) break;
@@ -385,16 +425,7 @@ __device__ inline void render_line(int x, const SceneDev& sc, const rlevox_rayma
       }
     }
   }
-  if (COUNT) {
-    atomicAdd(counts + 0, 1ull);
-    atomicAdd(counts + 1, (unsigned long long)cnt.steps);
-    atomicAdd(counts + 2, (unsigned long long)cnt.ptr);
-    atomicAdd(counts + 3, (unsigned long long)cnt.rle);
-    atomicAdd(counts + 4, (unsigned long long)cnt.elems);
-    atomicAdd(counts + 5, (unsigned long long)cnt.spans);
-    atomicAdd(counts + 6, (unsigned long long)cnt.col);
-    atomicAdd(counts + 7, (unsigned long long)cnt.cleared);
-  }
+  if (COUNT) cnt.flush(counts);
 }
 
 // One screen pixel of the compose pass (colorize_buddha_soft.frag:12-92);

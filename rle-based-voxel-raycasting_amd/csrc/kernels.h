@@ -16,6 +16,9 @@ hipError_t launch_raycast_plain(const SceneDev& sc, const rlevox_raymap& rm, uin
                                 unsigned long long* counts = nullptr);
 hipError_t launch_compose(const rlevox_raymap& rm, const uint32_t* raybuf, const int32_t* clip,
                           uint16_t* col, int32_t* dep, hipStream_t s);
+// Smallest world row of a drawn run's top over all columns of one level,
+// atomically min-ed into *out (which must hold INT32_MAX or a larger row).
+hipError_t launch_column_tops(const MipDev& m, int level, int32_t* out, hipStream_t s);
 hipError_t launch_gapfill(int W, int H, const uint16_t* cin, const int32_t* dep, uint16_t* cout,
                           hipStream_t s);
 

@@ -40,7 +40,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                                      unsigned long long* counts) {
   static_assert(kDdaLds * 4 <= 64 * RC * 2, "DDA scratch must fit in the run cache");
   __shared__ uint4 rcache[64 * RC / 8];
-  __shared__ int sweep[64];
+  __shared__ int sweep[64 + kWalkEndWords];
   float* dda = reinterpret_cast<float*>(rcache);  // see kDdaLds
   extern __shared__ uint32_t lds[];
   if ((int)blockIdx.x >= rm.map_line_count) return;
@@ -57,7 +57,7 @@ __global__ __launch_bounds__(128) void k_raycast_wave2(SceneDev sc, rlevox_rayma
                                                        uint32_t* __restrict__ raybuf,
                                                        int32_t* __restrict__ clip) {
   __shared__ uint4 rcache[64 * RC / 8];
-  __shared__ int sweep[64];
+  __shared__ int sweep[64 + kWalkEndWords];
   __shared__ float dda[kDdaLds];
   __shared__ ChunkRing ring;
   extern __shared__ uint32_t lds[];
@@ -134,6 +134,44 @@ hipError_t launch_raycast_plain(const SceneDev& sc, const rlevox_raymap& rm, uin
   else
     hipLaunchKernelGGL(k_raycast_plain<false>, dim3(grid), dim3(block), lds, s, sc, rm, raybuf,
                        clip, counts);
+  return hipGetLastError();
+}
+
+// The world row of each column's first drawn run (the element walk of
+// Cuda_Render.h:511-524 up to the first element with drawn voxels), reduced
+// to the minimum over the level.  Columns are validated at upload, so the
+// walk stays inside the slab array.
+__global__ __launch_bounds__(256) void k_column_tops(const uint2* __restrict__ map,
+                                                     const uint16_t* __restrict__ slabs, int shift,
+                                                     int level, uint64_t ncols, int32_t* out) {
+  int best = INT32_MAX;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t c = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; c < ncols; c += stride) {
+    const uint2 e = map[c];
+    const uint32_t n = e.y & 0xffffu;
+    const uint16_t* runs = slabs + 2 + ((uint64_t)e.x << shift);
+    uint32_t w = e.y >> 16;
+    int y = 0;
+    for (uint32_t k = 0; k < n; k++) {
+      if (k > 0) w = runs[k];
+      y += (int)(w & 1023u) << level;
+      if (w >> 10) {
+        best = min(best, y);
+        break;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) best = min(best, __shfl_xor(best, o));
+  if ((threadIdx.x & 63) == 0 && best != INT32_MAX) atomicMin(out, best);
+}
+
+hipError_t launch_column_tops(const MipDev& m, int level, int32_t* out, hipStream_t s) {
+  const uint64_t ncols = (uint64_t)m.sx * (uint64_t)m.sz;
+  if (ncols == 0) return hipSuccess;
+  uint64_t blocks = (ncols + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_column_tops, dim3((unsigned)blocks), dim3(256), 0, s, m.map, m.slabs,
+                     m.shift, level, ncols, out);
   return hipGetLastError();
 }
 

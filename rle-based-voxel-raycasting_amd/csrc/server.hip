@@ -11,6 +11,13 @@
 //          captured hipGraph of reset + raycast + compose + gap-fill),
 //          a device framebuffer arena and a pinned host staging area.
 //
+// A batch completes as a whole, so a small request packed with a large one
+// waits for it.  With large_lanes > 0 the first lanes take only large
+// requests (W*H >= large_pixels), with a persistent raycast of large_waves
+// wavefronts per CU so the GPU keeps room for the others, and the remaining
+// lanes take only the smaller requests; batch_pixels caps the pixels of one
+// batch, which bounds how long it holds its lane.
+//
 // Admission is by device memory: a request holds its ray buffer, line clip
 // ranges, gap-fill temporary and framebuffers from launch to completion, and
 // a batch only takes requests while the bytes in flight stay within
@@ -56,6 +63,7 @@ struct Req {
   const rlevox_scene* scene = nullptr;
   rlevox_pose pose{};
   int W = 0, H = 0;
+  bool large = false;            // served by the large lanes (large_lanes > 0)
   uint16_t* d_colour = nullptr;  // caller's device framebuffer (device delivery)
   int32_t* d_depth = nullptr;
   size_t bytes = 0;  // device bytes held while in flight
@@ -94,6 +102,7 @@ struct Lane {
   std::vector<Req*> reqs;
   std::vector<size_t> fb_off;  // per host-delivered request: colour at off, depth after it
   size_t bytes = 0;
+  bool large = false;     // takes only large requests (else only the others)
   bool busy = false;
   bool finished = false;  // set by the completion host function
   Completion done{nullptr, nullptr};
@@ -259,66 +268,79 @@ void worker_main(rlevox_server* srv) {
       progress = true;
     }
     if (progress) srv->cv_done.notify_all();
-    Lane* free_lane = nullptr;
-    for (auto& lp : srv->lanes)
-      if (!lp->busy) {
-        free_lane = lp.get();
-        break;
-      }
     if (srv->stop && srv->inflight_batches == 0) break;
-    if (!srv->stop && free_lane && !srv->queue.empty() && srv->fatal.empty()) {
-      // pack queued requests in arrival order while memory allows
-      std::vector<Req*> take;
-      size_t bytes = 0;
-      for (auto it = srv->queue.begin(); it != srv->queue.end() &&
-                                         (int)take.size() < srv->cfg.max_batch_views;) {
-        Req* r = *it;
-        if (r->cancel) {
-          r->state = CANCELLED;
+    // pack queued requests of a free lane's class in arrival order while
+    // memory, the batch's view count and its pixel cap allow
+    Lane* free_lane = nullptr;
+    std::vector<Req*> take;
+    size_t bytes = 0;
+    if (!srv->stop && !srv->queue.empty() && srv->fatal.empty()) {
+      const bool split = srv->cfg.large_lanes > 0;
+      for (auto& lp : srv->lanes) {
+        if (lp->busy) continue;
+        uint64_t pixels = 0;
+        for (auto it = srv->queue.begin(); it != srv->queue.end() &&
+                                           (int)take.size() < srv->cfg.max_batch_views;) {
+          Req* r = *it;
+          if (r->cancel) {
+            r->state = CANCELLED;
+            it = srv->queue.erase(it);
+            continue;
+          }
+          if (split && r->large != lp->large) {
+            ++it;
+            continue;
+          }
+          const uint64_t px = (uint64_t)r->W * (uint64_t)r->H;
+          if (!take.empty() && srv->cfg.batch_pixels && pixels + px > srv->cfg.batch_pixels) break;
+          if (srv->inflight_bytes + bytes + r->bytes > srv->cfg.memory_budget) break;
+          bytes += r->bytes;
+          pixels += px;
+          take.push_back(r);
           it = srv->queue.erase(it);
-          continue;
         }
-        if (srv->inflight_bytes + bytes + r->bytes > srv->cfg.memory_budget) break;
-        bytes += r->bytes;
-        take.push_back(r);
-        it = srv->queue.erase(it);
+        if (!take.empty()) {
+          free_lane = lp.get();
+          break;
+        }
       }
-      if (!take.empty()) {
-        for (Req* r : take) r->state = RUNNING;
-        free_lane->reqs = take;
-        free_lane->bytes = bytes;
-        free_lane->busy = true;
-        free_lane->finished = false;
-        srv->inflight_bytes += bytes;
-        if (srv->inflight_batches++ == 0) srv->busy_since = Clock::now();
-        srv->stats.batches++;
-        rlevox_tuning tuning;
-        {
-          std::lock_guard<std::mutex> tl(srv->ctx->mu);
-          tuning = srv->ctx->tuning;
-        }
+    }
+    if (!take.empty()) {
+      for (Req* r : take) r->state = RUNNING;
+      free_lane->reqs = take;
+      free_lane->bytes = bytes;
+      free_lane->busy = true;
+      free_lane->finished = false;
+      srv->inflight_bytes += bytes;
+      if (srv->inflight_batches++ == 0) srv->busy_since = Clock::now();
+      srv->stats.batches++;
+      rlevox_tuning tuning;
+      {
+        std::lock_guard<std::mutex> tl(srv->ctx->mu);
+        tuning = srv->ctx->tuning;
+      }
+      if (free_lane->large && srv->cfg.large_waves > 0) tuning.persistent_waves = srv->cfg.large_waves;
+      lock.unlock();
+      rlevox_status st = launch_lane(srv, *free_lane, tuning);
+      std::string msg = st == RLEVOX_OK ? std::string() : std::string(rlevox_last_error());
+      lock.lock();
+      if (st != RLEVOX_OK) {
+        // nothing of this batch is pending on the device unless the launch
+        // itself went through; wait for the lane's stream before reuse
         lock.unlock();
-        rlevox_status st = launch_lane(srv, *free_lane, tuning);
-        std::string msg = st == RLEVOX_OK ? std::string() : std::string(rlevox_last_error());
+        hipStreamSynchronize(free_lane->slot.stream);
         lock.lock();
-        if (st != RLEVOX_OK) {
-          // nothing of this batch is pending on the device unless the launch
-          // itself went through; wait for the lane's stream before reuse
-          lock.unlock();
-          hipStreamSynchronize(free_lane->slot.stream);
-          lock.lock();
-          fail_requests(srv, free_lane->reqs, st, msg);
-          srv->inflight_bytes -= free_lane->bytes;
-          free_lane->bytes = 0;
-          free_lane->reqs.clear();
-          free_lane->busy = false;
-          if (--srv->inflight_batches == 0)
-            srv->stats.busy_seconds += ms_since(srv->busy_since, Clock::now()) * 1e-3;
-          if (st == RLEVOX_ERR_DEVICE) srv->fatal = msg;
-          srv->cv_done.notify_all();
-        }
-        continue;
+        fail_requests(srv, free_lane->reqs, st, msg);
+        srv->inflight_bytes -= free_lane->bytes;
+        free_lane->bytes = 0;
+        free_lane->reqs.clear();
+        free_lane->busy = false;
+        if (--srv->inflight_batches == 0)
+          srv->stats.busy_seconds += ms_since(srv->busy_since, Clock::now()) * 1e-3;
+        if (st == RLEVOX_ERR_DEVICE) srv->fatal = msg;
+        srv->cv_done.notify_all();
       }
+      continue;
     }
     if (!srv->fatal.empty()) {  // fail whatever is queued
       std::vector<Req*> rest(srv->queue.begin(), srv->queue.end());
@@ -348,6 +370,12 @@ rlevox_status rlevox_server_create(rlevox_ctx* ctx, const rlevox_server_config* 
   if (cfg->max_queue < 1) return fail(RLEVOX_ERR_INVALID_ARG, "server_create: max_queue < 1");
   if (cfg->streams < 1 || cfg->streams > 8)
     return fail(RLEVOX_ERR_INVALID_ARG, "server_create: streams must be 1..8");
+  if (cfg->large_lanes < 0 || (cfg->large_lanes > 0 && cfg->large_lanes >= cfg->streams))
+    return fail(RLEVOX_ERR_INVALID_ARG, "server_create: large_lanes must be 0..streams-1");
+  if (cfg->large_lanes > 0 && cfg->large_pixels < 1)
+    return fail(RLEVOX_ERR_INVALID_ARG, "server_create: large_lanes needs large_pixels >= 1");
+  if (cfg->large_waves != 0 && (cfg->large_waves < 4 || cfg->large_waves > 32))
+    return fail(RLEVOX_ERR_INVALID_ARG, "server_create: large_waves must be 0 or 4..32");
   const int post = post_flags(&cfg->params);
   if (post & ~(RLEVOX_POST_COMPOSE | RLEVOX_POST_GAPFILL) || !(post & RLEVOX_POST_COMPOSE))
     return fail(RLEVOX_ERR_INVALID_ARG, "server_create: post flags must include compose");
@@ -359,6 +387,7 @@ rlevox_status rlevox_server_create(rlevox_ctx* ctx, const rlevox_server_config* 
   srv->post = post;
   for (int i = 0; i < cfg->streams; i++) {
     srv->lanes.emplace_back(new Lane());
+    srv->lanes.back()->large = i < cfg->large_lanes;
     if ((st = srv->lanes.back()->slot.init(ctx)) != RLEVOX_OK) {
       for (auto& l : srv->lanes) l->slot.release();
       delete srv;
@@ -401,6 +430,7 @@ rlevox_status rlevox_server_submit(rlevox_server* srv, const rlevox_scene* scene
   r->pose = *pose;
   r->W = W;
   r->H = H;
+  r->large = srv->cfg.large_lanes > 0 && (int64_t)W * H >= srv->cfg.large_pixels;
   r->d_colour = d_colour;
   r->d_depth = d_depth;
   r->bytes = bytes;

@@ -22,6 +22,14 @@ const rlevox_tuning kCandidates[] = {
     {32, 2, 0, 1},    {16, 1, 0, 1},    {16, 4, 0, 1},
 };
 
+// The plain autotuner leaves the batched path's settings as they are.
+void keep_batch_fields(rlevox_tuning& t, const rlevox_tuning& saved) {
+  t.persistent_waves = saved.persistent_waves;
+  t.order = saved.order;
+  t.batch_run_cache = saved.batch_run_cache;
+  t.batch_par = saved.batch_par;
+}
+
 rlevox_status time_views(rlevox_ctx* ctx, const rlevox_view* views, int n,
                          const rlevox_render_params* params, hipStream_t s, hipEvent_t a,
                          hipEvent_t b, int reps, double* ms) {
@@ -86,8 +94,7 @@ extern "C" rlevox_status rlevox_autotune(rlevox_ctx* ctx, const rlevox_view* vie
     int best_i = -1;
     for (int c = 0; c < ncand; c++) {
       rlevox_tuning t = kCandidates[c];
-      t.persistent_waves = saved.persistent_waves;
-      t.order = saved.order;
+      keep_batch_fields(t, saved);
       {
         std::lock_guard<std::mutex> lock(ctx->mu);
         ctx->tuned[keys[g]] = t;
@@ -105,8 +112,7 @@ extern "C" rlevox_status rlevox_autotune(rlevox_ctx* ctx, const rlevox_view* vie
     }
     if (st != RLEVOX_OK) break;
     rlevox_tuning t = kCandidates[best_i];
-    t.persistent_waves = saved.persistent_waves;
-    t.order = saved.order;
+    keep_batch_fields(t, saved);
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->tuned[keys[g]] = t;
     if (g == 0) {
@@ -123,6 +129,81 @@ extern "C" rlevox_status rlevox_autotune(rlevox_ctx* ctx, const rlevox_view* vie
       ctx->tuning = saved;
     }
   }
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  if (s) hipStreamDestroy(s);
+  return st;
+}
+
+extern "C" rlevox_status rlevox_autotune_batch(rlevox_ctx* ctx, const rlevox_view* views,
+                                               int32_t nviews, const rlevox_render_params* params,
+                                               rlevox_tuning* best, double* best_ms) {
+  clear_error();
+  rlevox_status st = ctx_bind(ctx);
+  if (st != RLEVOX_OK) return st;
+  if (!views || nviews < 1 || nviews > BatchSlot::kMaxViews || !best)
+    return fail(RLEVOX_ERR_INVALID_ARG, "autotune_batch: need 1..%d views and an output",
+                BatchSlot::kMaxViews);
+  rlevox_tuning saved;
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    saved = ctx->tuning;
+  }
+  hipStream_t s = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&b);
+  if (e != hipSuccess) st = hip_fail(e, "autotune_batch setup");
+  // the batch shape: persistent wavefronts per CU, view order, run-cache
+  // depth and element replay, all combinations
+  const int kWaves[] = {8, 12, 16, 24}, kOrder[] = {0, 1}, kRc[] = {24, 40, 56}, kPar[] = {1, 2};
+  rlevox_tuning win = saved;
+  double win_ms = 0;
+  bool have = false;
+  for (int w : kWaves)
+    for (int o : kOrder)
+      for (int rc : kRc)
+        for (int par : kPar) {
+          if (st != RLEVOX_OK) break;
+          rlevox_tuning t = saved;
+          t.persistent_waves = w;
+          t.order = o;
+          t.batch_run_cache = rc;
+          t.batch_par = par;
+          {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            ctx->tuning = t;
+          }
+          // one untimed batch (code-object load, scratch growth), then two timed
+          double ms = 0;
+          for (int rep = 0; rep < 3 && st == RLEVOX_OK; rep++) {
+            if (rep == 1) e = hipEventRecord(a, s);
+            if (e == hipSuccess) st = rlevox_render_batch(ctx, views, nviews, params, s);
+            else st = hip_fail(e, "autotune_batch event");
+          }
+          if (st != RLEVOX_OK) break;
+          e = hipEventRecord(b, s);
+          if (e == hipSuccess) e = hipEventSynchronize(b);
+          float tm = 0;
+          if (e == hipSuccess) e = hipEventElapsedTime(&tm, a, b);
+          if (e != hipSuccess) {
+            st = hip_fail(e, "autotune_batch timing");
+            break;
+          }
+          ms = (double)tm / 2;
+          if (!have || ms < win_ms) {
+            win = t;
+            win_ms = ms;
+            have = true;
+          }
+        }
+  {
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->tuning = st == RLEVOX_OK ? win : saved;
+    if (st == RLEVOX_OK) *best = win;
+  }
+  if (st == RLEVOX_OK && best_ms) *best_ms = win_ms;
   if (a) hipEventDestroy(a);
   if (b) hipEventDestroy(b);
   if (s) hipStreamDestroy(s);

@@ -451,6 +451,9 @@ __device__ inline void ring_get(const ChunkRing& r, int s, DdaChunk& c, int lane
 // precomputes a column's remaining elements across the lanes (pays where
 // steps often show several elements, as in grazing views of caved terrain;
 // the persistent batch kernel leaves it off, where its registers spill).
+// LDS arguments: bits = the line's bitmap, dl = DDA scratch, rc = run cache,
+// sweep_lds = 64 ints for the horizon sweep followed by kWalkEndWords words
+// for the early-end test's constants.
 template <bool COUNT, bool TWO = false, bool PAR_ELEMS = true, int RC = kRunCache>
 __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_raymap& rm,
                                         uint32_t* __restrict__ row, int32_t* __restrict__ clip,
@@ -597,6 +600,12 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
   const float y_add = (vertical ? cos_x : 0.0f) * hmul;
   const int zfar = rm.far_distance;
   const float mountain = vpy;
+  // the early-end test's constants, in LDS behind the sweep marks
+  WalkEnd* wend = reinterpret_cast<WalkEnd*>(sweep_lds + 64);
+  if (!producer) {
+    if (lane == 0) wend->init(cos_x, sin_x, hmul, ray_x, ray_z, vertical, mountain, half);
+    wave_sync();
+  }
 
   // Software pipeline: while chunk c's run-cache loads are in flight the DDA
   // of chunk c+1 runs, and while chunk c is replayed the pointer-map gather
@@ -1111,6 +1120,12 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
       }
       cur_step = t + 1;
     }
+    // ---- early end of the walk (WalkEnd): later steps lie beyond this
+    // chunk's last crossing distance
+    if (!done && !cnt.ended && wend->done(readlane_f(cur.dn, L - 1), ymin, sc.mip[mip].top, sc.sy)) {
+      if (COUNT) cnt.end_walk();  // the counting variant walks on to count the reference's work
+      else done = true;
+    }
     return !done;
   };
   if constexpr (!TWO) {
@@ -1182,16 +1197,7 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
       }
     }
   }
-  if (COUNT && lane == 0) {
-    atomicAdd(counts + 0, 1ull);
-    atomicAdd(counts + 1, (unsigned long long)cnt.steps);
-    atomicAdd(counts + 2, (unsigned long long)cnt.ptr);
-    atomicAdd(counts + 3, (unsigned long long)cnt.rle);
-    atomicAdd(counts + 4, (unsigned long long)cnt.elems);
-    atomicAdd(counts + 5, (unsigned long long)cnt.spans);
-    atomicAdd(counts + 6, (unsigned long long)cnt.col);
-    atomicAdd(counts + 7, (unsigned long long)cnt.cleared);
-  }
+  if (COUNT && lane == 0) cnt.flush(counts);
 }
 
 }  // namespace rlevox

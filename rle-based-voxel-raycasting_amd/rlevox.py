@@ -77,11 +77,22 @@ class RayMap(C.Structure):
 
 
 class Counters(C.Structure):
-    _fields_ = [(n, C.c_uint64) for n in ("lines", "dda_steps", "ptr_loads", "rle_loads", "elems",
-                                          "spans", "col_loads", "cleared")]
+    REFERENCE = ("lines", "dda_steps", "ptr_loads", "rle_loads", "elems", "spans", "col_loads",
+                 "cleared")
+    WALK = ("walk_steps", "walk_ptr_loads", "walk_rle_loads", "walk_elems")
+    _fields_ = [(n, C.c_uint64) for n in REFERENCE + WALK]
 
     def as_dict(self):
-        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+        """Work of the reference's walk (every line to its far distance)."""
+        return {n: int(getattr(self, n)) for n in self.REFERENCE}
+
+    def walk_dict(self):
+        """Work of the kernels' walk (lines end once nothing more can be
+        drawn), in the same keys as as_dict()."""
+        d = self.as_dict()
+        for n, k in zip(self.WALK, ("dda_steps", "ptr_loads", "rle_loads", "elems")):
+            d[k] = int(getattr(self, n))
+        return d
 
 
 class GenParams(C.Structure):
@@ -98,17 +109,21 @@ class View(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("lines_per_wave", C.c_int32), ("waves_per_block", C.c_int32),
-                ("persistent_waves", C.c_int32), ("order", C.c_int32), ("run_cache", C.c_int32)]
+                ("persistent_waves", C.c_int32), ("order", C.c_int32), ("run_cache", C.c_int32),
+                ("batch_run_cache", C.c_int32), ("batch_par", C.c_int32)]
 
     def as_tuple(self):
-        return (self.lines_per_wave, self.waves_per_block, self.persistent_waves, self.order,
-                self.run_cache)
+        return tuple(getattr(self, n) for n, _ in self._fields_)
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class ServerConfig(C.Structure):
     _fields_ = [("memory_budget", C.c_uint64), ("max_batch_views", C.c_int32),
-                ("max_queue", C.c_int32), ("streams", C.c_int32), ("reserved", C.c_int32),
-                ("params", RenderParams)]
+                ("max_queue", C.c_int32), ("streams", C.c_int32), ("large_lanes", C.c_int32),
+                ("params", RenderParams), ("batch_pixels", C.c_uint64),
+                ("large_pixels", C.c_int64), ("large_waves", C.c_int32), ("reserved", C.c_int32)]
 
 
 class ServerStats(C.Structure):
@@ -156,6 +171,8 @@ _sigs = {
     "rlevox_get_tuning_for": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, P(Tuning)]),
     "rlevox_debug_fail_alloc": (None, [C.c_int32]),
     "rlevox_autotune": (C.c_int, [_vp, P(View), C.c_int32, P(RenderParams), P(Tuning), P(C.c_double)]),
+    "rlevox_autotune_batch": (C.c_int, [_vp, P(View), C.c_int32, P(RenderParams), P(Tuning),
+                                        P(C.c_double)]),
     "rlevox_server_create": (C.c_int, [_vp, P(ServerConfig), P(_vp)]),
     "rlevox_server_submit": (C.c_int, [_vp, _vp, P(Pose), C.c_int32, C.c_int32, _vp, _vp,
                                        P(C.c_uint64)]),
@@ -420,8 +437,9 @@ class Context:
         return float(ms.value)
 
     def set_tuning(self, lines_per_wave=1, waves_per_block=0, persistent_waves=0, order=1,
-                   run_cache=0):
-        t = Tuning(lines_per_wave, waves_per_block, persistent_waves, order, run_cache)
+                   run_cache=0, batch_run_cache=0, batch_par=0):
+        t = Tuning(lines_per_wave, waves_per_block, persistent_waves, order, run_cache,
+                   batch_run_cache, batch_par)
         check(lib.rlevox_set_tuning(self.handle, C.byref(t)))
 
     def get_tuning(self):
@@ -440,11 +458,13 @@ class Context:
         check(lib.rlevox_render_raybuf(self.handle, scene.handle, C.byref(rm), d_raybuf.ptr,
                                        d_clip.ptr, stream))
 
-    def raycast_counters(self, scene, rm, d_raybuf, d_clip):
+    def raycast_counters(self, scene, rm, d_raybuf, d_clip, walk=False):
+        """Exact work counts of one frame: the reference's walk, or with
+        walk=True the kernels' own (lines end once nothing more can draw)."""
         c = Counters()
         check(lib.rlevox_raycast_counters(self.handle, scene.handle, C.byref(rm), d_raybuf.ptr,
                                           d_clip.ptr, C.byref(c)))
-        return c.as_dict()
+        return c.walk_dict() if walk else c.as_dict()
 
     def compose(self, rm, d_raybuf, d_clip, d_col, d_dep, stream=None):
         check(lib.rlevox_compose(self.handle, C.byref(rm), d_raybuf.ptr, d_clip.ptr, d_col.ptr,
@@ -473,15 +493,28 @@ class Context:
                                   C.byref(ms)))
         return best, ms.value
 
+    def autotune_batch(self, views, params=None):
+        """Times the batch of `views` over the batch shapes (persistent
+        wavefronts per CU, view order, run-cache depth, element replay) and
+        makes the fastest the context's batch settings."""
+        arr = (View * len(views))(*views)
+        prm = params or RenderParams(0, 0, 0, 0)
+        best = Tuning()
+        ms = C.c_double()
+        check(lib.rlevox_autotune_batch(self.handle, arr, len(views), C.byref(prm),
+                                        C.byref(best), C.byref(ms)))
+        return best, ms.value
+
 
 class Server:
     """Continuous-batching render loop (rlevox_server_*).  submit() returns a
     ticket; wait() returns (colour, depth, latency_ms) as numpy arrays."""
 
     def __init__(self, ctx, memory_budget=8 << 30, max_batch_views=64, max_queue=4096, streams=2,
-                 params=None):
-        cfg = ServerConfig(memory_budget, max_batch_views, max_queue, streams, 0,
-                           params or RenderParams(0, 0, 0, 0))
+                 params=None, large_lanes=0, batch_pixels=0, large_pixels=0, large_waves=0):
+        cfg = ServerConfig(memory_budget, max_batch_views, max_queue, streams, large_lanes,
+                           params or RenderParams(0, 0, 0, 0), batch_pixels, large_pixels,
+                           large_waves, 0)
         h = C.c_void_p()
         check(lib.rlevox_server_create(ctx.handle, C.byref(cfg), C.byref(h)))
         self.handle = h

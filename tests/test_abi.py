@@ -53,7 +53,7 @@ def test_struct_layouts_match_c(tmp_path):
 
 
 def test_abi_version():
-    assert rlevox.lib.rlevox_abi_version() == 2
+    assert rlevox.lib.rlevox_abi_version() == 3
 
 
 def test_no_device_is_reported_not_crashed():
@@ -69,3 +69,15 @@ def test_null_arguments_are_rejected():
     assert st == 1 and "null" in rlevox.last_error()
     st = rlevox.lib.rlevox_render(None, None, None, 640, 480, None, None, None, None)
     assert st == 1
+
+
+def test_counter_dicts():
+    """as_dict() holds the reference's walk, walk_dict() the same keys with
+    the kernels' walk values (the fields a walk that ends early changes)."""
+    c = rlevox.Counters(*range(1, 13))
+    ref = c.as_dict()
+    assert list(ref) == list(rlevox.Counters.REFERENCE) and ref["cleared"] == 8
+    walk = c.walk_dict()
+    assert set(walk) == set(ref)
+    assert (walk["dda_steps"], walk["ptr_loads"], walk["rle_loads"], walk["elems"]) == (9, 10, 11, 12)
+    assert all(walk[k] == ref[k] for k in ("lines", "spans", "col_loads", "cleared"))

@@ -112,6 +112,52 @@ def test_batch_graph_replays(ctx, scenes, oracle):
         free_views(views)
 
 
+def test_batch_shapes_equal_plain(ctx, scenes):
+    """Every batch shape the batch autotuner can choose (run-cache depth,
+    element replay, persistent wavefronts per CU, view order) renders the
+    same framebuffers as plain renders."""
+    hosts, devs = scenes
+    views = make_views(ctx, hosts, devs, 12, seed=41)
+    for rc, par, waves, order in ((24, 1, 8, 0), (56, 2, 24, 1), (40, 2, 12, 0), (56, 1, 16, 1)):
+        ctx.set_tuning(persistent_waves=waves, order=order, batch_run_cache=rc, batch_par=par)
+        for v in views:
+            v["dc"].fill_u32(0)
+            v["dd"].fill_u32(0)
+        ctx.render_batch([v["view"] for v in views])
+        check_views(ctx, devs, views, tag="batch rc %d par %d waves %d" % (rc, par, waves))
+    ctx.set_tuning()
+    free_views(views)
+
+
+def test_batch_autotune(ctx, scenes, oracle):
+    """The batch autotuner picks a batch shape, makes it the context's, and
+    batches rendered with it (eager and graph) equal plain renders and, on a
+    sample, the oracle."""
+    hosts, devs = scenes
+    views = make_views(ctx, hosts, devs, 16, seed=43)
+    try:
+        best, ms = ctx.autotune_batch([v["view"] for v in views])
+        assert ms > 0
+        t = ctx.get_tuning()
+        assert t.as_tuple() == best.as_tuple()
+        assert best.persistent_waves in (8, 12, 16, 24) and best.order in (0, 1)
+        assert best.batch_run_cache in (24, 40, 56) and best.batch_par in (1, 2)
+        for graph in (False, True):
+            for v in views:
+                v["dc"].fill_u32(0)
+                v["dd"].fill_u32(0)
+            ctx.render_batch([v["view"] for v in views], graph=graph)
+            check_views(ctx, devs, views, tag="tuned batch graph=%s" % graph)
+        for v in views[:3]:
+            W, H = v["W"], v["H"]
+            ref = oracle.render(hosts[v["si"]].mips, v["pose"], W, H)
+            assert np.array_equal(v["dc"].download(np.uint16, W * H).reshape(H, W), ref["colour"])
+            assert np.array_equal(v["dd"].download(np.int32, W * H).reshape(H, W), ref["depth"])
+    finally:
+        ctx.set_tuning()
+        free_views(views)
+
+
 def test_batch_rejects_bad_views(ctx, scenes):
     hosts, devs = scenes
     d = ctx.alloc(64)

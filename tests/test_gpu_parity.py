@@ -118,6 +118,15 @@ def test_c1_pitch_and_altitude_edges(ctx, oracle, c1_scene, c1_dev):
         check_view(ctx, oracle, c1_dev, c1_scene.mips, p, 512, 384)
 
 
+def test_c1_views_among_the_hills(ctx, oracle, c1_scene, c1_dev):
+    """Eye 2-20 voxels above the ground, below the highest runs: most lines
+    end their walk early (no later step can draw); every kernel still gives
+    the oracle's full walk bit for bit."""
+    for k, p in enumerate(workloads.random_views(c1_scene.mips[0], 4, seed=12, alt=(2.0, 20.0))):
+        W, H = ((640, 480), (1280, 720))[k % 2]
+        check_view(ctx, oracle, c1_dev, c1_scene.mips, p, W, H)
+
+
 def test_c1_mip_and_far_distance(ctx, oracle, c1_scene, c1_dev):
     p = workloads.random_views(c1_scene.mips[0], 1, seed=99)[0]
     for mipd, far in [(64, 5000), (300, 80000), (4096, 20000), (4, 100)]:
@@ -147,8 +156,13 @@ def test_render_entry_matches_stages(ctx, oracle, c1_scene, c1_dev):
 
 
 def test_work_counters_match_oracle(ctx, oracle, c1_scene, c1_dev):
-    """The counting kernel's exact work counts equal the oracle's."""
-    for p in workloads.flythrough(c1_scene.mips[0], frames=3, seed=11):
+    """The counting kernel's exact work counts equal the oracle's.  The
+    flythrough's eye stays above the terrain's highest run, where no line can
+    end before the far distance; the low views (eye among the hills) end
+    most of their lines early."""
+    ended = 0
+    low = workloads.random_views(c1_scene.mips[0], 3, seed=12, alt=(2.0, 20.0))
+    for p in workloads.flythrough(c1_scene.mips[0], frames=3, seed=11) + low:
         rm = rlevox.raymap(p, 1280, 720)
         lines = max(rm.map_line_count, 1)
         d_rb = ctx.alloc(lines * rm.rs * 4)
@@ -158,9 +172,18 @@ def test_work_counters_match_oracle(ctx, oracle, c1_scene, c1_dev):
             ctx.set_tuning(lines_per_wave=lpw, waves_per_block=wpb, run_cache=rc)
             got = ctx.raycast_counters(c1_dev, rm, d_rb, d_clip)
             assert got == want, (lpw, got, want)
+            # the kernels' own walk (lines end once nothing more can draw):
+            # the same drawing, fewer steps and loads
+            walk = ctx.raycast_counters(c1_dev, rm, d_rb, d_clip, walk=True)
+            for k in ("lines", "spans", "col_loads", "cleared"):
+                assert walk[k] == want[k], (lpw, k)
+            for k in ("dda_steps", "ptr_loads", "rle_loads", "elems"):
+                assert walk[k] <= want[k], (lpw, k)
+            ended += want["dda_steps"] - walk["dda_steps"]
         ctx.set_tuning()
         d_rb.free()
         d_clip.free()
+    assert ended > 0  # some lines of these views end before the far distance
 
 
 def test_golden_digests(ctx, c1_scene, tiny_scene, c1_dev, tiny_dev):

@@ -79,6 +79,41 @@ def test_served_results_equal_plain_renders(ctx, scenes, c1_scene, tiny_scene, o
         srv.close()
 
 
+def test_large_lanes_and_batch_pixel_cap(ctx, scenes, c1_scene, tiny_scene, oracle):
+    """Large requests on a lane of their own (fewer persistent wavefronts),
+    the others on the remaining lanes with a pixel cap per batch: every
+    framebuffer equals the plain render, a sample equals the oracle, and no
+    batch mixes the two classes or exceeds the cap (beyond its first view)."""
+    reqs = requests(c1_scene, tiny_scene, 30, seed=33)
+    hosts = [c1_scene, tiny_scene]
+    big = 1280 * 720
+    srv = rlevox.Server(ctx, memory_budget=2 << 30, max_batch_views=16, streams=3,
+                        large_lanes=1, large_pixels=big, large_waves=8,
+                        batch_pixels=1920 * 1080)
+    try:
+        ts = [srv.submit(scenes[si], p, W, H) for si, p, W, H in reqs]
+        for k, ((si, p, W, H), t) in enumerate(zip(reqs, ts)):
+            col, dep, _ = srv.wait(t, timeout_ms=60000)
+            want = plain_render(ctx, scenes[si], p, W, H)
+            assert np.array_equal(col, want[0]) and np.array_equal(dep, want[1]), k
+            if k % 6 == 1:
+                ref = oracle.render(hosts[si].mips, p, W, H)
+                assert np.array_equal(col, ref["colour"]) and np.array_equal(dep, ref["depth"]), k
+        st = srv.stats()
+        assert st["completed"] == len(reqs)
+        # the pixel cap alone forces at least this many batches
+        small = sum(W * H for _, _, W, H in reqs if W * H < big)
+        assert st["batches"] >= -(-small // (1920 * 1080))
+    finally:
+        srv.close()
+    for bad in (dict(large_lanes=3), dict(large_lanes=1), dict(large_waves=2)):
+        kw = dict(streams=3, large_pixels=0)
+        kw.update(bad)
+        with pytest.raises(rlevox.RlevoxError) as e:
+            rlevox.Server(ctx, **kw)
+        assert e.value.name == "INVALID_ARG"
+
+
 def test_alone_and_in_a_mix_are_identical(ctx, scenes, c1_scene, tiny_scene):
     reqs = requests(c1_scene, tiny_scene, 24, seed=5)
     probe = reqs[7]

@@ -1,7 +1,9 @@
 """Native host checks (CPU): the wavefront kernel's chunked DDA against the
-sequential walk, and the library's host code plus the oracle under
-AddressSanitizer and UndefinedBehaviorSanitizer."""
+sequential walk, the early end of a line's walk against the oracle's full
+walk, and the library's host code plus the oracle under AddressSanitizer and
+UndefinedBehaviorSanitizer."""
 import os
+import re
 import subprocess
 
 import pytest
@@ -24,6 +26,38 @@ def test_chunked_dda_matches_sequential_walk(tmp_path):
     rc, out = build_and_run("dda_chunk_check", 3000, tmp_path)
     assert rc == 0, out
     assert "mismatches 0" in out
+
+
+def _walk_end_run(tmp_path, plant):
+    exe = str(tmp_path / ("walk_end_check_%d" % plant))
+    srcs = [os.path.join(NATIVE, "walk_end_check.cpp")]
+    srcs += [os.path.join(CSRC, f) for f in ("host.cpp", "raymap.cpp", "scene.cpp")]
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fopenmp", "-pthread",
+                    "-Wno-subobject-linkage", "-DWALK_END_PLANT=%d" % plant] + srcs + ["-o", exe],
+                   check=True)
+    r = subprocess.run([exe, "40"], stdout=subprocess.PIPE, text=True,
+                       env=dict(os.environ, OMP_NUM_THREADS="4"))
+    assert r.returncode == 0, r.stdout
+    m = re.search(r"lines ended early (\d+), steps walked (\d+) of (\d+), violations (\d+)",
+                  r.stdout)
+    assert m, r.stdout
+    return [int(v) for v in m.groups()]
+
+
+def test_walk_end_never_cuts_a_draw(tmp_path):
+    """WalkEnd (the early end of a line's walk) evaluated before every step
+    of the oracle's full walk on 80 random views: no step after the claimed
+    end draws or moves the window, and the walk needs far fewer steps."""
+    ended, walked, full, violations = _walk_end_run(tmp_path, 0)
+    assert violations == 0
+    assert ended > 1000 and walked < 0.8 * full
+
+
+@pytest.mark.parametrize("plant", [1, 4])
+def test_walk_end_check_catches_a_wrong_top(tmp_path, plant):
+    """The same check with every run top taken `plant` rows too low (a bound
+    that is too optimistic) finds steps that still draw."""
+    assert _walk_end_run(tmp_path, plant)[3] > 0
 
 
 def test_host_code_under_sanitizers(tmp_path):
