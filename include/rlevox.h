@@ -322,6 +322,23 @@ rlevox_status rlevox_render(rlevox_ctx* ctx, const rlevox_scene* scene, const rl
                             int32_t W, int32_t H, const rlevox_render_params* params,
                             uint16_t* d_colour, int32_t* d_depth, void* stream);
 
+/* rlevox_render without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph
+ * capture, serving loops that forbid hipMalloc).  Its scratch (ray buffer,
+ * line clip ranges, gap-fill temporary) is a fixed array of
+ * RLEVOX_NOALLOC_SCRATCH bytes that each GPU the library runs on holds until
+ * the process ends.  A view needs 4*W bytes per ray line, 8 bytes per line
+ * and, with gap-fill, 2*W*H bytes (each rounded up to 256); one needing more
+ * than the array fails with OVER_BUDGET (the 1920x1080 flythrough views
+ * need 20-30 MB).  Stream-ordered: it enqueues the frame
+ * on `stream` and returns without waiting.  All calls on one GPU share that
+ * GPU's array, so calls that overlap on one GPU race. */
+#define RLEVOX_NOALLOC_SCRATCH (64u << 20)
+rlevox_status rlevox_render_noalloc(rlevox_ctx* ctx, const rlevox_scene* scene,
+                                    const rlevox_pose* pose, int32_t W, int32_t H,
+                                    const rlevox_render_params* params, uint16_t* d_colour,
+                                    int32_t* d_depth, void* stream);
+
 /* Many views (any scenes, any sizes) in one persistent launch over a device
  * work queue; each framebuffer is identical to rlevox_render's. */
 rlevox_status rlevox_render_batch(rlevox_ctx* ctx, const rlevox_view* views, int32_t nviews,

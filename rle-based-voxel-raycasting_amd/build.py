@@ -18,7 +18,7 @@ OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librlevox.so")
 ARCH = os.environ.get("RLEVOX_ARCH", "gfx950")
 
-HIP_SOURCES = ["kernels.hip", "api.hip", "batch.hip", "server.hip", "tune.hip"]
+HIP_SOURCES = ["kernels.hip", "api.hip", "batch.hip", "server.hip", "tune.hip", "noalloc.hip"]
 HOST_SOURCES = ["host.cpp", "raymap.cpp", "scene.cpp"]
 
 FP_FLAGS = ["-ffp-contract=off", "-fno-fast-math"]

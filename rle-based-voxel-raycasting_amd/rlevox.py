@@ -164,6 +164,8 @@ _sigs = {
     "rlevox_compose": (C.c_int, [_vp, P(RayMap), _vp, _vp, _vp, _vp, _vp]),
     "rlevox_gapfill": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp, _vp, _vp, _vp]),
     "rlevox_render": (C.c_int, [_vp, _vp, P(Pose), C.c_int32, C.c_int32, P(RenderParams), _vp, _vp, _vp]),
+    "rlevox_render_noalloc": (C.c_int, [_vp, _vp, P(Pose), C.c_int32, C.c_int32, P(RenderParams),
+                                        _vp, _vp, _vp]),
     "rlevox_render_batch": (C.c_int, [_vp, P(View), C.c_int32, P(RenderParams), _vp]),
     "rlevox_render_batch_graph": (C.c_int, [_vp, P(View), C.c_int32, P(RenderParams), _vp]),
     "rlevox_set_tuning": (C.c_int, [_vp, P(Tuning)]),
@@ -477,6 +479,13 @@ class Context:
         prm = params or RenderParams(0, 0, 0, 0)
         check(lib.rlevox_render(self.handle, scene.handle, C.byref(pose), W, H, C.byref(prm),
                                 d_col.ptr, d_dep.ptr, stream))
+
+    def render_noalloc(self, scene, pose, W, H, d_col, d_dep, params=None, stream=None):
+        """rlevox_render_noalloc: the same frame without any device allocation
+        (fixed per-GPU scratch); stream-ordered, returns before it is done."""
+        prm = params or RenderParams(0, 0, 0, 0)
+        check(lib.rlevox_render_noalloc(self.handle, scene.handle, C.byref(pose), W, H,
+                                        C.byref(prm), d_col.ptr, d_dep.ptr, stream))
 
     def render_batch(self, views, params=None, stream=None, graph=False):
         arr = (View * len(views))(*views)
