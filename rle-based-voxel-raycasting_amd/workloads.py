@@ -39,7 +39,7 @@ def surface_row(mip0, x, z):
     """Row of the first stored voxel of column (x, z), or sy if empty."""
     sx, sy, sz = mip0["sx"], mip0["sy"], mip0["sz"]
     c = (int(z) % sz) * sx + (int(x) % sx)
-    ofs = int(mip0["ptrmap"][2 * c])
+    ofs = int(mip0["ptrmap"][2 * c]) << int(mip0.get("shift", 0))  # widened offsets
     slabs = mip0["slabs"]
     count = int(slabs[ofs])
     y = 0

@@ -229,6 +229,11 @@ def test_repack_with_offset_shift_keeps_the_scene(oracle, shift):
     for k in ("raybuf", "clip", "colour", "depth"):
         assert np.array_equal(ra[k], rb[k]), k
     assert ra["counters"] == rb["counters"]
+    # camera placement reads the widened offsets too: same surface rows, same poses
+    m0a, m0b = sc.mips[0], rp.mips[0]
+    for x, z in ((3.2, 7.9), (40.5, 12.1), (63.9, 63.9), (0.0, 30.3)):
+        assert W.surface_row(m0a, x, z) == W.surface_row(m0b, x, z)
+    assert (W.random_views(m0b, 3, seed=9)[2].pos[:] == W.random_views(m0a, 3, seed=9)[2].pos[:])
     back = rlevox.repack(rp.mips, 0)  # and back to the reference layout
     for a, b in zip(sc.mips, back.mips):
         assert np.array_equal(a["slabs"], b["slabs"]) and np.array_equal(a["ptrmap"], b["ptrmap"])
