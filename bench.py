@@ -100,11 +100,11 @@ def main():
                     help="make K of the 12 resident scenes device copies of the "
                          "16384^2 x 1024 terrain (~33 GB each) instead of 4096^2 ones "
                          "(default 3: ~105 GB resident)")
-    ap.add_argument("--serve-large-lanes", type=int, default=1,
+    ap.add_argument("--serve-large-lanes", type=int, default=0,
                     help="server lanes reserved for 3840x2160 requests (0: any lane takes any)")
     ap.add_argument("--serve-large-waves", type=int, default=8,
                     help="persistent raycast wavefronts per CU on the large lanes")
-    ap.add_argument("--serve-batch-mpix", type=float, default=8.0,
+    ap.add_argument("--serve-batch-mpix", type=float, default=0.0,
                     help="pixel cap of one served batch, in millions (0: none)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--waves-per-line", type=int, choices=(0, 1, 2), default=0,
