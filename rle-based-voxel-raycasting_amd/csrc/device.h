@@ -546,37 +546,58 @@ __device__ inline uint16_t gapfill_pixel(int W, int H, const uint16_t* __restric
 // (larger radii) read their taps from memory.  Call from all 256 threads.
 constexpr int kGapHalo = 12;
 constexpr int kGapTW = 64 + 2 * kGapHalo, kGapTH = 4 + 2 * kGapHalo;
+// One staged pixel: its depth, and its tap word (shade byte | 1 << 16 |
+// colour high byte << 24).  Summing the tap words of the qualifying taps
+// counts them in bits 16-23 and adds their shades in bits 0-15 (at most 26
+// pixels of 255), whatever the high bytes add above.
 struct GapTile {
-  int32_t d[kGapTH][kGapTW];
-  uint16_t c[kGapTH][kGapTW];
+  uint2 p[kGapTH][kGapTW];
 };
+
+__device__ inline uint32_t gap_word(uint32_t c) {
+  return (c & 0xffu) | (1u << 16) | ((c >> 8) << 24);
+}
 
 __device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ cin,
                                     const int32_t* __restrict__ dep, uint16_t* __restrict__ cout,
                                     int x0, int y0, GapTile& t) {
-  // staging: every load of the thread is issued before the first store, so
-  // the loads' latencies overlap
-  constexpr int kStage = (kGapTW * kGapTH + 255) / 256;
-  int32_t dv[kStage];
-  uint16_t cv[kStage];
   // uniform: a tile whose border lies inside the image needs no clamping
   const bool inner = x0 >= kGapHalo && y0 >= kGapHalo && x0 - kGapHalo + kGapTW <= W &&
                      y0 - kGapHalo + kGapTH <= H;
-  if (inner) {
+  if (inner && (W & 3) == 0) {
+    // four pixels per load (x0 - kGapHalo and W are multiples of 4); every
+    // load of the thread is issued before the first store
+    constexpr int kQ = kGapTW / 4, kQuads = kQ * kGapTH, kStage = (kQuads + 255) / 256;
+    static_assert(kGapTW % 4 == 0 && kGapHalo % 4 == 0, "quads must align");
     const size_t g0 = (size_t)(y0 - kGapHalo) * W + (size_t)(x0 - kGapHalo);
-    const int32_t* dt = dep + g0;
-    const uint16_t* ct = cin + g0;
+    int4 dv[kStage];
+    uint2 cv[kStage];
 #pragma unroll
     for (int k = 0; k < kStage; k++) {
       const int i = threadIdx.x + k * 256;
-      if (i < kGapTW * kGapTH) {
-        const int ty = i / kGapTW, tx = i - ty * kGapTW;
-        const int o = ty * W + tx;  // < kGapTH * W
-        dv[k] = dt[o];
-        cv[k] = ct[o];
+      if (i < kQuads) {
+        const int ty = i / kQ, tq = i - ty * kQ;
+        const size_t o = g0 + (size_t)ty * W + 4 * tq;
+        dv[k] = *reinterpret_cast<const int4*>(dep + o);
+        cv[k] = *reinterpret_cast<const uint2*>(cin + o);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kStage; k++) {
+      const int i = threadIdx.x + k * 256;
+      if (i < kQuads) {
+        const int ty = i / kQ, tq = i - ty * kQ;
+        uint4* dst = reinterpret_cast<uint4*>(&t.p[ty][4 * tq]);
+        dst[0] = make_uint4((uint32_t)dv[k].x, gap_word(cv[k].x & 0xffffu), (uint32_t)dv[k].y,
+                            gap_word(cv[k].x >> 16));
+        dst[1] = make_uint4((uint32_t)dv[k].z, gap_word(cv[k].y & 0xffffu), (uint32_t)dv[k].w,
+                            gap_word(cv[k].y >> 16));
       }
     }
   } else {
+    constexpr int kStage = (kGapTW * kGapTH + 255) / 256;
+    int32_t dv[kStage];
+    uint16_t cv[kStage];
 #pragma unroll
     for (int k = 0; k < kStage; k++) {
       const int i = threadIdx.x + k * 256;
@@ -589,24 +610,23 @@ __device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ c
         cv[k] = cin[g];
       }
     }
-  }
 #pragma unroll
-  for (int k = 0; k < kStage; k++) {
-    const int i = threadIdx.x + k * 256;
-    if (i < kGapTW * kGapTH) {
-      const int ty = i / kGapTW, tx = i - ty * kGapTW;
-      t.d[ty][tx] = dv[k];
-      t.c[ty][tx] = cv[k];
+    for (int k = 0; k < kStage; k++) {
+      const int i = threadIdx.x + k * 256;
+      if (i < kGapTW * kGapTH) {
+        const int ty = i / kGapTW, tx = i - ty * kGapTW;
+        t.p[ty][tx] = make_uint2((uint32_t)dv[k], gap_word(cv[k]));
+      }
     }
   }
   __syncthreads();
   const int lx = (threadIdx.x & 63) + kGapHalo, ly = (threadIdx.x >> 6) + kGapHalo;
   const int x = x0 + (threadIdx.x & 63), prow = y0 + (threadIdx.x >> 6);
   if (x < W && prow < H) {
-    const uint16_t c0 = t.c[ly][lx];
-    int dmin = t.d[ly][lx];
-    for (int k = 0; k < 7; k++) dmin = min(dmin, t.d[ly - kRingDy[k]][lx + kRingDx[k]]);
-    uint16_t out = c0;
+    const uint2 ctr = t.p[ly][lx];
+    int dmin = (int)ctr.x;
+    for (int k = 0; k < 7; k++) dmin = min(dmin, (int)t.p[ly - kRingDy[k]][lx + kRingDx[k]].x);
+    uint16_t out = (uint16_t)((ctr.y & 0xffu) | ((ctr.y >> 24) << 8));  // the pixel's colour
     if (dmin < RLEVOX_SKY_DEPTH) {
       const int dm = max(dmin, 1);
       int r_q8 = (int)(78720.0f / (float)dm);  // floor(78720 / dm), corrected below
@@ -620,27 +640,26 @@ __device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ c
           o[a] = (int)((uint32_t)((2 * a - 5) * r_q8 + 640 + 655360) / 1280u) - 512;
         if (o[0] >= -kGapHalo) {
           // r_q8 >= 42 means dmin <= 1874: a tap qualifies when
-          // 10*dmin >= 7*d, i.e. d <= floor(10*dmin/7) (which is below the sky depth)
-          // Branch-free: every tap's depth and colour are read (so all 50
-          // LDS reads can be in flight together) and a failing tap adds 0.
-          int n = 1, sum = c0 & 0xff;
+          // 10*dmin >= 7*d, i.e. d <= floor(10*dmin/7) (below the sky depth).
+          // Branch-free: every tap is read (so the 25 LDS reads can be in
+          // flight together) and a failing tap adds 0.  The centre pixel
+          // counts once with its own shade.
           const int dlim = (10 * dmin) / 7;
+          uint32_t acc = ctr.y;
           int rowo[5];
 #pragma unroll
           for (int b = 0; b < 5; b++) rowo[b] = (ly - o[b]) * kGapTW + lx;
+          const uint2* tp = &t.p[0][0];
 #pragma unroll
           for (int a = 0; a < 5; a++) {
 #pragma unroll
             for (int b = 0; b < 5; b++) {
-              const int i = rowo[b] + o[a];
-              const int d = (&t.d[0][0])[i];
-              const int c = (&t.c[0][0])[i] & 0xff;
-              const bool q = d <= dlim;
-              n += q ? 1 : 0;
-              sum += q ? c : 0;
+              const uint2 q = tp[rowo[b] + o[a]];
+              acc += (int)q.x <= dlim ? q.y : 0u;
             }
           }
-          if (n > 6) out = (uint16_t)((c0 & 0xff00) | ((sum + n / 2) / n));
+          const int n = (int)((acc >> 16) & 0xffu), sum = (int)(acc & 0xffffu);
+          if (n > 6) out = (uint16_t)((out & 0xff00) | ((sum + n / 2) / n));
         } else {
           out = gapfill_pixel(W, H, cin, dep, x, prow);
         }
