@@ -500,7 +500,8 @@ def bench_served(ctx, rlevox, workloads, scenes_h, scenes, args):
                 "p50_ms": round(float(np.percentile(lat, 50)), 3),
                 "p99_ms": round(float(np.percentile(lat, 99)), 3),
                 "p99_1080p_ms": round(float(np.percentile(lat[r1080], 99)), 3),
-                "batches": int(st["batches"]), "busy_s": round(st["busy_seconds"], 3)}
+                "batches": int(st["batches"]), "busy_s": round(st["busy_seconds"], 3),
+                "buffer_growths": int(st["buffer_growths"])}
 
     run(None)  # warm-up: lane graphs, scratch growth
     sat = run(None)

@@ -224,6 +224,8 @@ typedef struct {
   uint64_t submitted, completed, rejected, batches;
   uint64_t rays;                 /* W*H summed over completed requests */
   double busy_seconds;           /* wall time with at least one batch in flight */
+  uint64_t buffer_growths;       /* lane buffers replaced by larger ones (the
+                                    old ones are freed once nothing is in flight) */
 } rlevox_server_stats;
 
 /* ------------------------------------------------------------ utilities */

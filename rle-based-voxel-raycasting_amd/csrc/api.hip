@@ -41,9 +41,11 @@ rlevox_status ctx_bind(rlevox_ctx* ctx) {
 rlevox_status Scratch::ensure(size_t bytes) {
   if (bytes <= size) return RLEVOX_OK;
   if (ptr) {
-    hipFree(ptr);
+    if (retire) retire->push_back(ptr);
+    else hipFree(ptr);
     ptr = nullptr;
     size = 0;
+    grown++;
   }
   size_t want = bytes + bytes / 4;
   hipError_t e = dev_malloc(&ptr, want);

@@ -17,6 +17,11 @@ namespace rlevox {
 struct Scratch {
   void* ptr = nullptr;
   size_t size = 0;
+  // When set, a buffer replaced by a larger one goes to this list instead of
+  // hipFree (which waits for the whole device): its owner frees the list
+  // when nothing is in flight.
+  std::vector<void*>* retire = nullptr;
+  uint64_t grown = 0;  // replacements by a larger buffer
   rlevox_status ensure(size_t bytes);
   void release();
 };

@@ -119,8 +119,12 @@ __global__ __launch_bounds__(256) void k_gapfill_batch(const BatchHeader* __rest
   __shared__ GapTile gt;
   const int64_t total = h->total_tiles;
   const int nv = h->nviews;
-  int v = find_view<true>(views, nv, (int64_t)blockIdx.x);
-  for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+  // Each round of gridDim.x tiles is dealt to the 8 XCDs in eight contiguous
+  // runs (blocks go round robin over the XCDs; gridDim.x is a multiple of
+  // 8), so neighbouring tiles, whose borders overlap, share an L2.
+  const int64_t first = (int64_t)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  int v = find_view<true>(views, nv, first);
+  for (int64_t tile = first; tile < total; tile += gridDim.x) {
     while (v + 1 < nv && views[v + 1].tile_begin <= tile) v++;  // tiles only grow
     const ViewDesc& vd = views[v];  // a few fields per tile: read in place
     if (!(vd.post & RLEVOX_POST_GAPFILL)) continue;  // uniform over the block

@@ -99,6 +99,11 @@ struct Lane {
   Scratch fb;  // colour + depth of the batch's host-delivered requests
   void* pinned = nullptr;
   size_t pinned_size = 0;
+  // Buffers replaced by larger ones while other lanes may have work in
+  // flight: freed when no batch is in flight (hipFree and hipHostFree wait
+  // for the whole device, which would stall every lane).
+  std::vector<void*> retired_dev, retired_host;
+  uint64_t pinned_grown = 0;
   std::vector<Req*> reqs;
   std::vector<size_t> fb_off;  // per host-delivered request: colour at off, depth after it
   size_t bytes = 0;
@@ -161,7 +166,10 @@ rlevox_status launch_lane(rlevox_server* srv, Lane& lane, const rlevox_tuning& t
   rlevox_status st = fb_bytes ? lane.fb.ensure(fb_bytes) : RLEVOX_OK;
   if (st != RLEVOX_OK) return st;
   if (fb_bytes && lane.pinned_size < fb_bytes) {
-    if (lane.pinned) hipHostFree(lane.pinned);
+    if (lane.pinned) {
+      lane.retired_host.push_back(lane.pinned);
+      lane.pinned_grown++;
+    }
     lane.pinned = nullptr;
     lane.pinned_size = 0;
     const size_t want = fb_bytes + fb_bytes / 4;
@@ -230,6 +238,17 @@ void finish_lane(rlevox_server* srv, Lane& lane) {
     srv->stats.busy_seconds += ms_since(srv->busy_since, now) * 1e-3;
 }
 
+// Frees the buffers lanes replaced by larger ones.  Called by the worker
+// with no batch in flight (or at destroy).
+void free_retired(rlevox_server* srv) {
+  for (auto& lp : srv->lanes) {
+    for (void* p : lp->retired_dev) hipFree(p);
+    for (void* p : lp->retired_host) hipHostFree(p);
+    lp->retired_dev.clear();
+    lp->retired_host.clear();
+  }
+}
+
 void fail_requests(rlevox_server* srv, const std::vector<Req*>& rs, rlevox_status st,
                    const std::string& msg) {
   for (Req* r : rs) {
@@ -261,6 +280,7 @@ void worker_main(rlevox_server* srv) {
       }
     }
     if (progress) srv->cv_done.notify_all();
+    if (srv->inflight_batches == 0) free_retired(srv);  // nothing in flight: no stall
     // drop cancelled requests at the head of the queue
     while (!srv->queue.empty() && srv->queue.front()->cancel) {
       srv->queue.front()->state = CANCELLED;
@@ -387,7 +407,10 @@ rlevox_status rlevox_server_create(rlevox_ctx* ctx, const rlevox_server_config* 
   srv->post = post;
   for (int i = 0; i < cfg->streams; i++) {
     srv->lanes.emplace_back(new Lane());
-    srv->lanes.back()->large = i < cfg->large_lanes;
+    Lane& ln = *srv->lanes.back();
+    ln.large = i < cfg->large_lanes;
+    ln.fb.retire = &ln.retired_dev;
+    ln.slot.scratch.retire = &ln.retired_dev;
     if ((st = srv->lanes.back()->slot.init(ctx)) != RLEVOX_OK) {
       for (auto& l : srv->lanes) l->slot.release();
       delete srv;
@@ -520,6 +543,9 @@ rlevox_status rlevox_server_stats_get(rlevox_server* srv, rlevox_server_stats* o
   std::lock_guard<std::mutex> lock(srv->mu);
   *out = srv->stats;
   if (srv->inflight_batches > 0) out->busy_seconds += ms_since(srv->busy_since, Clock::now()) * 1e-3;
+  out->buffer_growths = 0;
+  for (auto& lp : srv->lanes)
+    out->buffer_growths += lp->fb.grown + lp->slot.scratch.grown + lp->pinned_grown;
   return RLEVOX_OK;
 }
 
@@ -535,6 +561,7 @@ void rlevox_server_destroy(rlevox_server* srv) {
   srv->cv_done.notify_all();
   if (srv->worker.joinable()) srv->worker.join();  // the worker drains in-flight batches first
   ctx_bind(srv->ctx);
+  free_retired(srv);
   for (auto& l : srv->lanes) {
     l->slot.release();
     l->fb.release();

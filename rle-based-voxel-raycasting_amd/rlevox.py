@@ -128,7 +128,8 @@ class ServerConfig(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("rejected", C.c_uint64),
-                ("batches", C.c_uint64), ("rays", C.c_uint64), ("busy_seconds", C.c_double)]
+                ("batches", C.c_uint64), ("rays", C.c_uint64), ("busy_seconds", C.c_double),
+                ("buffer_growths", C.c_uint64)]
 
 
 def _load():

@@ -75,6 +75,7 @@ def test_served_results_equal_plain_renders(ctx, scenes, c1_scene, tiny_scene, o
         assert st["batches"] >= 5  # at most 8 views per batch
         assert st["rays"] == sum(W * H for _, _, W, H in reqs)
         assert st["busy_seconds"] > 0
+        assert st["buffer_growths"] >= 1  # lanes grew their buffers for larger batches
     finally:
         srv.close()
 
