@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device.h"
+#include "tile_queue.h"
 
 namespace rlevox {
 
@@ -30,6 +31,8 @@ struct BatchHeader {
   int32_t max_words;  // bitmap words of the largest view
   uint32_t queue;     // next line to hand out (reset by k_batch_reset)
   uint32_t pad;
+  TileQueueWords compose_q;  // tile queues (reset by k_batch_reset)
+  TileQueueWords gapfill_q;
 };
 
 // par_elems: the raycast replays a column's elements with the lane-parallel

@@ -18,6 +18,10 @@ namespace rlevox {
 
 __global__ void k_batch_reset(BatchHeader* h) {
   if (threadIdx.x == 0) h->queue = 0;
+  if (threadIdx.x < kTileQueues) {
+    h->compose_q.w[threadIdx.x].v = 0;
+    h->gapfill_q.w[threadIdx.x].v = 0;
+  }
 }
 
 // Largest i with begin(i) <= g (views are ordered by begin).
@@ -72,6 +76,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   }
 }
 
+struct DeviceAdd {
+  __device__ uint64_t operator()(uint64_t* p, uint64_t v) const {
+    return atomicAdd((unsigned long long*)p, (unsigned long long)v);
+  }
+};
+
+// The view holding `tile`, from the view v of the workgroup's previous
+// tile: mostly v or the next view (the home queue's tiles only grow), a
+// search otherwise (the first tile, tiles taken from other queues).
+__device__ inline int view_of_tile(const ViewDesc* __restrict__ views, int nv, int v, int64_t tile) {
+  if (v + 1 < nv && views[v + 1].tile_begin <= tile) {
+    v++;
+    if (v + 1 < nv && views[v + 1].tile_begin <= tile) v = find_view<true>(views, nv, tile);
+  } else if (tile < views[v].tile_begin) {
+    v = find_view<true>(views, nv, tile);
+  }
+  return v;
+}
+
 // The descriptor of the view a block is working on, copied to LDS when the
 // block's tiles move to another view (its tiles only grow, so views change
 // rarely): compose's per-tile reads of the ray map's fields then come from
@@ -85,14 +108,22 @@ __device__ inline void stage_view(const ViewDesc* __restrict__ views, int v, Vie
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_compose_batch(const BatchHeader* __restrict__ h,
+__global__ __launch_bounds__(256) void k_compose_batch(BatchHeader* __restrict__ h,
                                                        const ViewDesc* __restrict__ views) {
   __shared__ ViewDesc vd;
-  const int64_t total = h->total_tiles;
+  __shared__ int64_t next[2];
   const int nv = h->nviews;
-  int v = find_view<true>(views, nv, (int64_t)blockIdx.x), staged = -1;
-  for (int64_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
-    while (v + 1 < nv && views[v + 1].tile_begin <= tile) v++;  // tiles only grow
+  TileWalk<DeviceAdd> walk;
+  if (threadIdx.x == 0) walk.start(&h->compose_q, h->total_tiles, blockIdx.x & (kTileQueues - 1));
+  int v = 0, staged = -1;
+  for (int it = 0;; it ^= 1) {
+    // two slots: a slot is written again only after the barrier that
+    // follows every thread's read of it
+    if (threadIdx.x == 0) next[it] = walk.take();
+    __syncthreads();
+    const int64_t tile = next[it];
+    if (tile < 0) break;
+    v = view_of_tile(views, nv, v, tile);
     if (v != staged) {
       stage_view(views, v, vd);
       staged = v;
@@ -114,18 +145,20 @@ __global__ __launch_bounds__(256) void k_compose_batch(const BatchHeader* __rest
   }
 }
 
-__global__ __launch_bounds__(256) void k_gapfill_batch(const BatchHeader* __restrict__ h,
+__global__ __launch_bounds__(256) void k_gapfill_batch(BatchHeader* __restrict__ h,
                                                        const ViewDesc* __restrict__ views) {
   __shared__ GapTile gt;
-  const int64_t total = h->total_tiles;
+  __shared__ int64_t next[2];
   const int nv = h->nviews;
-  // Each round of gridDim.x tiles is dealt to the 8 XCDs in eight contiguous
-  // runs (blocks go round robin over the XCDs; gridDim.x is a multiple of
-  // 8), so neighbouring tiles, whose borders overlap, share an L2.
-  const int64_t first = (int64_t)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-  int v = find_view<true>(views, nv, first);
-  for (int64_t tile = first; tile < total; tile += gridDim.x) {
-    while (v + 1 < nv && views[v + 1].tile_begin <= tile) v++;  // tiles only grow
+  TileWalk<DeviceAdd> walk;
+  if (threadIdx.x == 0) walk.start(&h->gapfill_q, h->total_tiles, blockIdx.x & (kTileQueues - 1));
+  int v = 0;
+  for (int it = 0;; it ^= 1) {
+    if (threadIdx.x == 0) next[it] = walk.take();
+    __syncthreads();
+    const int64_t tile = next[it];
+    if (tile < 0) break;
+    v = view_of_tile(views, nv, v, tile);
     const ViewDesc& vd = views[v];  // a few fields per tile: read in place
     if (!(vd.post & RLEVOX_POST_GAPFILL)) continue;  // uniform over the block
     const int W = vd.rm.W;

@@ -1,6 +1,6 @@
 """Native host checks (CPU): the wavefront kernel's chunked DDA against the
 sequential walk, the early end of a line's walk against the oracle's full
-walk, and the library's host code plus the oracle under AddressSanitizer and
+walk, the batched post passes' tile queues, and the library's host code plus the oracle under AddressSanitizer and
 UndefinedBehaviorSanitizer."""
 import os
 import re
@@ -26,6 +26,32 @@ def test_chunked_dda_matches_sequential_walk(tmp_path):
     rc, out = build_and_run("dda_chunk_check", 3000, tmp_path)
     assert rc == 0, out
     assert "mismatches 0" in out
+
+
+def _tile_queue_run(tmp_path, defines):
+    exe = str(tmp_path / ("tile_queue_check" + "".join(d.replace("=", "_") for d in defines)))
+    subprocess.run(["g++", "-O2", "-std=c++17"] + ["-D" + d for d in defines] +
+                   [os.path.join(NATIVE, "tile_queue_check.cpp"), "-o", exe], check=True)
+    r = subprocess.run([exe], stdout=subprocess.PIPE, text=True)
+    m = re.search(r"tile queue walks (\d+), stripe (\d+), chunk (\d+), failures (\d+)", r.stdout)
+    assert m, r.stdout
+    return r.returncode, [int(v) for v in m.groups()], r.stdout
+
+
+@pytest.mark.parametrize("stripe,chunk", [(64, 1), (64, 3), (1024, 4), (1024, 16)])
+def test_tile_queues_hand_out_every_tile_once(tmp_path, stripe, chunk):
+    """The batched post passes' tile queues, walked by 1-2048 workgroups
+    under random interleavings of their requests on batches of 1 to 710400
+    tiles: every tile once, every walk ends."""
+    rc, (runs, got_stripe, got_chunk, failures), out = _tile_queue_run(
+        tmp_path, ["RLEVOX_TILE_STRIPE=%d" % stripe, "RLEVOX_TILE_CHUNK=%d" % chunk])
+    assert rc == 0 and failures == 0 and runs > 100, out
+    assert (got_stripe, got_chunk) == (stripe, chunk)
+
+
+def test_tile_queue_check_catches_a_wrong_back_index(tmp_path):
+    rc, (_, _, _, failures), _ = _tile_queue_run(tmp_path, ["TILE_QUEUE_PLANT=1"])
+    assert rc != 0 and failures > 0
 
 
 def _walk_end_run(tmp_path, plant):
