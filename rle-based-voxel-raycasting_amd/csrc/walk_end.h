@@ -68,8 +68,16 @@ struct WalkEnd {
     }
     const float py = vertical ? R * (cx * y - sx * rz * d) : R * rx * d;
     const float pya = fabsf(R) * (fabsf(cx * y) + fabsf(sx * rz * d) + fabsf(rx * d));
-    return half + py / pz + kRel * (fabsf(half) + fabsf(py) * (pza / pz) / pz + pya / pz);
+    // one division: the products' rounding (a few ulps) is far inside the margin
+    const float inv = 1.0f / pz;
+    const float q = py * inv;
+    return half + q + kRel * (fabsf(half) + fabsf(q) * (pza * inv) + pya * inv);
   }
+
+  // A value no bound falls below: while ymin + (1 - kAbs) <= floor_(), done()
+  // is false whatever the distance (the bound is at least the horizon
+  // limit), so a line looking at its horizon skips the corners.
+  RLEVOX_HD float floor_() const { return on ? lim : __builtin_inff(); }
 
   // True when no step at a distance >= D0 can draw while the floating
   // horizon is at ymin; top: highest run top (world row) of the levels still

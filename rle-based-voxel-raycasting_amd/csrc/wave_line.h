@@ -602,9 +602,11 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
   const float mountain = vpy;
   // the early-end test's constants, in LDS behind the sweep marks
   WalkEnd* wend = reinterpret_cast<WalkEnd*>(sweep_lds + 64);
+  float wfloor = 0.0f;  // held in a scalar register across the walk
   if (!producer) {
     if (lane == 0) wend->init(cos_x, sin_x, hmul, ray_x, ray_z, vertical, mountain, half);
     wave_sync();
+    wfloor = uni_f(wend->floor_());
   }
 
   // Software pipeline: while chunk c's run-cache loads are in flight the DDA
@@ -1122,7 +1124,8 @@ __device__ inline void render_line_wave(int x, const SceneDev& sc, const rlevox_
     }
     // ---- early end of the walk (WalkEnd): later steps lie beyond this
     // chunk's last crossing distance
-    if (!done && !cnt.ended && wend->done(readlane_f(cur.dn, L - 1), ymin, sc.mip[mip].top, sc.sy)) {
+    if (!done && !cnt.ended && wfloor < (float)ymin + (1.0f - WalkEnd::kAbs) &&
+        wend->done(readlane_f(cur.dn, L - 1), ymin, sc.mip[mip].top, sc.sy)) {
       if (COUNT) cnt.end_walk();  // the counting variant walks on to count the reference's work
       else done = true;
     }
