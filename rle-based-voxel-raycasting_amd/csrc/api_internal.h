@@ -131,21 +131,20 @@ struct rlevox_ctx {
     auto it = tuned.find(TuneKey{s, W, H});
     return it == tuned.end() ? tuning : it->second;
   }
-  // Lines per CU up to which the automatic choice walks each line with two
-  // wavefronts: the two-wavefront kernel keeps 12 lines per CU resident, the
-  // one-wavefront kernel 16.  A frame whose lines all fit at once is bound by
-  // its longest lines, where overlapping the DDA with the drawing pays
-  // (1080p flythrough, 2000-3300 lines: 13% faster with two); beyond that a
-  // second round of long lines costs more than the overlap gains (4K
-  // flythrough, up to 6600 lines: 8% faster with one).
-  static constexpr int kTwoWaveLinesPerCU = 14;
+  // Widest ray-buffer line (screen width) up to which the automatic choice
+  // walks each line with two wavefronts, one on the DDA and one drawing.
+  // Measured over the 100-pose flythrough of the 4096^2 terrain (mean
+  // raycast ms, two wavefronts / one with a 56-word run cache): 640x480
+  // 0.094 / 0.126, 1280x720 0.103 / 0.132, 1920x1080 0.123 / 0.148,
+  // 2048x1152 0.129 / 0.154 (two better at every line count, up to 4096
+  // lines), 2560x1440 0.187 / 0.168, 3840x2160 0.279 / 0.245.
+  static constexpr int kTwoWaveMaxWidth = 2048;
   // 1 selects the wavefront-per-line kernel, 2 its two-wavefront form;
   // otherwise lanes per block (16 or more), halved until the block's
   // per-lane bitmaps (ceil(rs/32) words each) fit the workgroup's LDS.
   int tuning_block(const rlevox_tuning& t, int lines, int rs) const {
     if (t.lines_per_wave == 1) {
-      if (t.waves_per_block == 0)
-        return lines <= kTwoWaveLinesPerCU * (num_cus > 0 ? num_cus : 256) ? 2 : 1;
+      if (t.waves_per_block == 0) return rs <= kTwoWaveMaxWidth ? 2 : 1;
       return t.waves_per_block >= 2 ? 2 : 1;
     }
     int block = t.lines_per_wave * t.waves_per_block;
