@@ -63,7 +63,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   for (;;) {
     uint32_t g = 0;
     if (lane == 0) g = atomicAdd(&h->queue, 1u);
-    g = (uint32_t)__builtin_amdgcn_readfirstlane((int)__shfl((int)g, 0));
+    g = (uint32_t)__builtin_amdgcn_readlane((int)g, 0);
     if ((int64_t)g >= total) break;
     while (v + 1 < nv && views[v + 1].line_begin <= (int64_t)g) v++;
     const ViewDesc& vd = views[v];

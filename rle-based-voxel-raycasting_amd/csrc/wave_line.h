@@ -95,26 +95,39 @@ __device__ inline int rank_in(const float* arr, float v, int guess) {
   return j;
 }
 
+// Wavefront scans with DPP lane moves (register reads across lanes, no LDS
+// round trip): row_shr:n (0x110 + n) inside each 16-lane row, then
+// row_bcast:15 (0x142, into rows 1 and 3) and row_bcast:31 (0x143, into
+// rows 2 and 3) across rows, wave_shr:1 (0x138) for the exclusive form.  A
+// lane whose source lies outside keeps `old`, the operation's identity.
+// Call with every lane of the wavefront active.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline int dpp_move(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, ROW_MASK, 0xf, false);
+}
+
 // Exclusive prefix maximum over the lanes of the wavefront (lane 0 gets
 // `init`).
 __device__ inline int prefix_max_excl(int v, int init, int lane) {
-  int incl = v;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int o = __shfl_up(incl, off);
-    if (lane >= off) incl = max(incl, o);
-  }
-  const int prev = __shfl_up(incl, 1);
+  v = max(v, dpp_move<0x111>(INT32_MIN, v));
+  v = max(v, dpp_move<0x112>(INT32_MIN, v));
+  v = max(v, dpp_move<0x114>(INT32_MIN, v));
+  v = max(v, dpp_move<0x118>(INT32_MIN, v));
+  v = max(v, dpp_move<0x142, 0xa>(INT32_MIN, v));
+  v = max(v, dpp_move<0x143, 0xc>(INT32_MIN, v));
+  const int prev = dpp_move<0x138>(INT32_MIN, v);
   return lane == 0 ? init : max(init, prev);
 }
 
 // Inclusive prefix sum of v over lanes 0..lane.
 __device__ inline int prefix_sum_incl(int v, int lane) {
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int o = __shfl_up(v, off);
-    if (lane >= off) v += o;
-  }
+  (void)lane;
+  v += dpp_move<0x111>(0, v);
+  v += dpp_move<0x112>(0, v);
+  v += dpp_move<0x114>(0, v);
+  v += dpp_move<0x118>(0, v);
+  v += dpp_move<0x142, 0xa>(0, v);
+  v += dpp_move<0x143, 0xc>(0, v);
   return v;
 }
 
