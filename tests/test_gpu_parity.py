@@ -98,7 +98,7 @@ def test_c1_flythrough_512(ctx, oracle, c1_scene, c1_dev):
         check_view(ctx, oracle, c1_dev, c1_scene.mips, p, 512, 512)
 
 
-@pytest.mark.parametrize("W,H", [(640, 480), (1280, 720), (1920, 1080), (320, 240)])
+@pytest.mark.parametrize("W,H", [(640, 480), (1280, 720), (1920, 1080), (320, 240), (2560, 1440)])
 def test_c1_resolutions(ctx, oracle, c1_scene, c1_dev, W, H):
     for p in workloads.random_views(c1_scene.mips[0], 2, seed=W + H):
         check_view(ctx, oracle, c1_dev, c1_scene.mips, p, W, H)
