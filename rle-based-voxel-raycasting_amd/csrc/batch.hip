@@ -40,9 +40,8 @@ __device__ inline int find_view(const ViewDesc* __restrict__ v, int n, int64_t g
 // Persistent: each wavefront takes the next line of the batch until the
 // queue is drained; every wavefront reaches the exit test each iteration.
 // Register use is capped for four wavefronts per SIMD (16 per CU, the
-// default launch): a few cold values spill, and the batch still runs 5%
-// faster than at the three wavefronts per SIMD its natural 152 registers
-// allow.  PAR: the replay prepares a column's elements across the lanes and
+// default launch): two cold values spill, and the batch runs faster than
+// at the three wavefronts per SIMD its natural registers allow.  PAR: the replay prepares a column's elements across the lanes and
 // visits only those that can act, as the plain kernels do; it spills more
 // and pays only where columns hold many runs (served mix with three 16384^2
 // scenes resident: 4165 -> 4769 Mrays/s; 64 views over 4096^2 terrains:
