@@ -109,7 +109,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--waves-per-line", type=int, choices=(0, 1, 2), default=0,
                     help="wavefronts walking each ray line (2: DDA and drawing overlap; "
-                         "0: the library's choice, two for frames at most 2048 pixels wide)")
+                         "0: the library's choice, two for frames at most 2560 pixels wide)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain copy kernel over MIB MiB (for counter calibration)")
     args = ap.parse_args()

@@ -135,10 +135,11 @@ struct rlevox_ctx {
   // walks each line with two wavefronts, one on the DDA and one drawing.
   // Measured over the 100-pose flythrough of the 4096^2 terrain (mean
   // raycast ms, two wavefronts / one with a 56-word run cache): 640x480
-  // 0.094 / 0.126, 1280x720 0.103 / 0.132, 1920x1080 0.123 / 0.148,
-  // 2048x1152 0.129 / 0.154 (two better at every line count, up to 4096
-  // lines), 2560x1440 0.187 / 0.168, 3840x2160 0.279 / 0.245.
-  static constexpr int kTwoWaveMaxWidth = 2048;
+  // 0.091 / 0.122, 1280x720 0.100 / 0.128, 1920x1080 0.118 / 0.141,
+  // 2048x1152 0.124 / 0.147, 2560x1440 0.142 / 0.160 (two better at every
+  // line count, up to 4096 and more lines), 3200x1800 0.210 / 0.186,
+  // 3840x2160 0.251 / 0.229.
+  static constexpr int kTwoWaveMaxWidth = 2560;
   // 1 selects the wavefront-per-line kernel, 2 its two-wavefront form;
   // otherwise lanes per block (16 or more), halved until the block's
   // per-lane bitmaps (ceil(rs/32) words each) fit the workgroup's LDS.
