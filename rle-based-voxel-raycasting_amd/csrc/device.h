@@ -502,20 +502,30 @@ __device__ inline void compose_pixel(const rlevox_raymap& rm, const uint32_t* __
 __constant__ const int kRingDx[7] = {0, 9, 9, 2, -8, -10, -3};
 __constant__ const int kRingDy[7] = {10, 6, -4, -10, -7, 3, 10};
 
+// Global-memory views of the gap-fill's buffers: the batch kernels read the
+// pointers from a view descriptor, where the compiler would emit flat loads.
+typedef __attribute__((address_space(1))) const int32_t g_i32;
+typedef __attribute__((address_space(1))) const uint16_t g_u16;
+typedef int v4i32 __attribute__((ext_vector_type(4)));
+typedef uint32_t v2u32 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const v4i32 g_v4i32;
+typedef __attribute__((address_space(1))) const v2u32 g_v2u32;
+typedef __attribute__((address_space(1))) uint16_t g_u16_out;
+
 __device__ inline int floor_div_pos(int a, int b) {  // b > 0
   const int q = a / b, r = a % b;
   return (r != 0 && r < 0) ? q - 1 : q;
 }
 
-__device__ inline uint16_t gapfill_pixel(int W, int H, const uint16_t* __restrict__ cin,
-                                         const int32_t* __restrict__ dep, int x, int prow) {
-  const size_t i = (size_t)prow * W + x;
+__device__ inline uint16_t gapfill_pixel(int W, int H, g_u16* __restrict__ cin,
+                                         g_i32* __restrict__ dep, int x, int prow) {
+  const uint32_t i = (uint32_t)prow * (uint32_t)W + (uint32_t)x;
   const uint16_t c0 = cin[i];
   int dmin = dep[i];
   for (int k = 0; k < 7; k++) {
     const int xx = min(max(x + kRingDx[k], 0), W - 1);
     const int yy = min(max(prow - kRingDy[k], 0), H - 1);
-    dmin = min(dmin, dep[(size_t)yy * W + xx]);
+    dmin = min(dmin, dep[(uint32_t)yy * (uint32_t)W + (uint32_t)xx]);
   }
   if (dmin >= RLEVOX_SKY_DEPTH) return c0;
   const int r_q8 = 78720 / max(dmin, 1);
@@ -527,7 +537,7 @@ __device__ inline uint16_t gapfill_pixel(int W, int H, const uint16_t* __restric
     for (int b = 0; b < 5; b++) {
       const int oy = floor_div_pos((2 * b - 5) * r_q8 + 640, 1280);
       const int yy = min(max(prow - oy, 0), H - 1);
-      const size_t t = (size_t)yy * W + xx;
+      const uint32_t t = (uint32_t)yy * (uint32_t)W + (uint32_t)xx;
       const int d = dep[t];
       if (d < RLEVOX_SKY_DEPTH && (int64_t)10 * dmin >= (int64_t)7 * d) {
         n++;
@@ -558,9 +568,19 @@ __device__ inline uint32_t gap_word(uint32_t c) {
   return (c & 0xffu) | (1u << 16) | ((c >> 8) << 24);
 }
 
-__device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ cin,
-                                    const int32_t* __restrict__ dep, uint16_t* __restrict__ cout,
+// floor(num / n) for 0 <= num < 2^13 and 1 <= n <= 26: num/n lies at least
+// 1/(2n) >= 1/52 from an integer once 1/(2n) is added, and the product with
+// the hardware reciprocal (1 ulp) is within 2^13 * 2^-21 = 1/256 of it.
+__device__ inline int small_div(int num, int n) {
+  return (int)(((float)num + 0.5f) * __builtin_amdgcn_rcpf((float)n));
+}
+
+__device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ cin_,
+                                    const int32_t* __restrict__ dep_, uint16_t* __restrict__ cout_,
                                     int x0, int y0, GapTile& t) {
+  // W, H <= 8192: pixel indices and byte offsets fit in 32 bits
+  g_u16* cin = (g_u16*)cin_;
+  g_i32* dep = (g_i32*)dep_;
   // uniform: a tile whose border lies inside the image needs no clamping
   const bool inner = x0 >= kGapHalo && y0 >= kGapHalo && x0 - kGapHalo + kGapTW <= W &&
                      y0 - kGapHalo + kGapTH <= H;
@@ -569,17 +589,17 @@ __device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ c
     // load of the thread is issued before the first store
     constexpr int kQ = kGapTW / 4, kQuads = kQ * kGapTH, kStage = (kQuads + 255) / 256;
     static_assert(kGapTW % 4 == 0 && kGapHalo % 4 == 0, "quads must align");
-    const size_t g0 = (size_t)(y0 - kGapHalo) * W + (size_t)(x0 - kGapHalo);
-    int4 dv[kStage];
-    uint2 cv[kStage];
+    const uint32_t g0 = (uint32_t)(y0 - kGapHalo) * (uint32_t)W + (uint32_t)(x0 - kGapHalo);
+    v4i32 dv[kStage];
+    v2u32 cv[kStage];
 #pragma unroll
     for (int k = 0; k < kStage; k++) {
       const int i = threadIdx.x + k * 256;
       if (i < kQuads) {
         const int ty = i / kQ, tq = i - ty * kQ;
-        const size_t o = g0 + (size_t)ty * W + 4 * tq;
-        dv[k] = *reinterpret_cast<const int4*>(dep + o);
-        cv[k] = *reinterpret_cast<const uint2*>(cin + o);
+        const uint32_t o = g0 + (uint32_t)ty * (uint32_t)W + 4u * (uint32_t)tq;
+        dv[k] = *(g_v4i32*)(dep + o);
+        cv[k] = *(g_v2u32*)(cin + o);
       }
     }
 #pragma unroll
@@ -605,7 +625,7 @@ __device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ c
         const int ty = i / kGapTW, tx = i - ty * kGapTW;
         const int gx = min(max(x0 - kGapHalo + tx, 0), W - 1);
         const int gy = min(max(y0 - kGapHalo + ty, 0), H - 1);
-        const size_t g = (size_t)gy * W + gx;
+        const uint32_t g = (uint32_t)gy * (uint32_t)W + (uint32_t)gx;
         dv[k] = dep[g];
         cv[k] = cin[g];
       }
@@ -629,8 +649,12 @@ __device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ c
     uint16_t out = (uint16_t)((ctr.y & 0xffu) | ((ctr.y >> 24) << 8));  // the pixel's colour
     if (dmin < RLEVOX_SKY_DEPTH) {
       const int dm = max(dmin, 1);
-      int r_q8 = (int)(78720.0f / (float)dm);  // floor(78720 / dm), corrected below
-      if (r_q8 * dm > 78720) r_q8--;
+      // floor(78720 / dm): the hardware reciprocal's estimate is within one
+      // of it (relative error below 2^-21 on a value below 78721), then the
+      // integer products settle it
+      int r_q8 = (int)(78720.0f * __builtin_amdgcn_rcpf((float)dm));
+      r_q8 += (r_q8 + 1) * dm <= 78720 ? 1 : 0;
+      r_q8 -= r_q8 * dm > 78720 ? 1 : 0;
       if (r_q8 >= 42) {
         // tap offsets floor(((2a-5) r + 640) / 1280); the numerators stay
         // above -655360, so a bias of 512*1280 keeps the division unsigned
@@ -659,13 +683,14 @@ __device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ c
             }
           }
           const int n = (int)((acc >> 16) & 0xffu), sum = (int)(acc & 0xffffu);
-          if (n > 6) out = (uint16_t)((out & 0xff00) | ((sum + n / 2) / n));
+          // n <= 26, sum + n/2 <= 26 * 255 + 13
+          if (n > 6) out = (uint16_t)((out & 0xff00) | small_div(sum + n / 2, n));
         } else {
           out = gapfill_pixel(W, H, cin, dep, x, prow);
         }
       }
     }
-    cout[(size_t)prow * W + x] = out;
+    *((g_u16_out*)cout_ + ((uint32_t)prow * (uint32_t)W + (uint32_t)x)) = out;
   }
   __syncthreads();  // the block's next tile reuses the LDS
 }
