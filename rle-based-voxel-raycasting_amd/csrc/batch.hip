@@ -110,6 +110,7 @@ __device__ inline void stage_view(const ViewDesc* __restrict__ views, int v, Vie
 __global__ __launch_bounds__(256) void k_compose_batch(BatchHeader* __restrict__ h,
                                                        const ViewDesc* __restrict__ views) {
   __shared__ ViewDesc vd;
+  __shared__ ComposeTile ct;
   __shared__ int64_t next[2];
   const int nv = h->nviews;
   TileWalk<DeviceAdd> walk;
@@ -131,12 +132,18 @@ __global__ __launch_bounds__(256) void k_compose_batch(BatchHeader* __restrict__
     const int local = (int)(tile - vd.tile_begin);  // < 2^18 tiles per view
     const int tiles_x = (W + 63) / 64;
     const int ty = local / tiles_x;
-    const int px = (local - ty * tiles_x) * 64 + (threadIdx.x & 63);
-    const int row = ty * 4 + (threadIdx.x >> 6);
+    const int x0 = (local - ty * tiles_x) * 64, y0 = ty * 4;
+    // the previous tile's reads of ct ended before this iteration's first
+    // barrier
+    compose_tile_stage(vd.rm, x0, y0, ct);
+    __syncthreads();
+    const int px = x0 + (threadIdx.x & 63);
+    const int row = y0 + (threadIdx.x >> 6);
     if (px < W && row < H) {
       uint16_t c;
       int32_t d;
-      compose_pixel(vd.rm, vd.raybuf, vd.clip, px, row, c, d);
+      compose_pixel(vd.rm, vd.raybuf, vd.clip, px, row, ct.col[threadIdx.x & 63],
+                    ct.row[threadIdx.x >> 6], ct.border, c, d);
       uint16_t* out = (vd.post & RLEVOX_POST_GAPFILL) ? vd.ctmp : vd.colour;
       out[(size_t)row * W + px] = c;
       vd.depth[(size_t)row * W + px] = d;

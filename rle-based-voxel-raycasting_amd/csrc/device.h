@@ -428,24 +428,72 @@ __device__ inline void render_line(int x, const SceneDev& sc, const rlevox_rayma
   if (COUNT) cnt.flush(counts);
 }
 
-// One screen pixel of the compose pass (colorize_buddha_soft.frag:12-92);
+// The compose pass (colorize_buddha_soft.frag:12-92) per screen pixel;
 // weights of exactly 0/1 are applied as selections.  Row 0 = image top.
+// The shader's quotients that depend on the column alone, on the row alone
+// or on neither are computed apart (the same operations on the same
+// operands), so that a tile can share them between its pixels.
+struct ComposeCol {
+  float scx;   // (px + 0.5) / RESX
+  float cterm; // |scx - vanish.x| * RESX / RESY
+};
+struct ComposeRow {
+  float scy;   // (H - 1 - prow + 0.5) / RESY
+  float ox;    // (H - 1 - prow + 0.5) / RESX
+};
+
+__device__ inline ComposeCol compose_col(const rlevox_raymap& rm, int px) {
+  const float RESX = (float)rm.W, RESY = (float)rm.H;
+  const float scx = ((float)px + 0.5f) / RESX;
+  return ComposeCol{scx, fabsf(scx - rm.vanish[0]) * RESX / RESY};
+}
+
+__device__ inline ComposeRow compose_row(const rlevox_raymap& rm, int prow) {
+  const float RESX = (float)rm.W, RESY = (float)rm.H;
+  const float fcy = (float)(rm.H - 1 - prow) + 0.5f;
+  return ComposeRow{fcy / RESY, fcy / RESX};
+}
+
+__device__ inline float compose_border(const rlevox_raymap& rm) {
+  const float RESX = (float)rm.W, RESY = (float)rm.H;
+  return (RESX - RESY) / (RESX * 2.0f);
+}
+
+// The shared quotients of one 64x4 tile (columns x0..x0+63, rows
+// y0..y0+3): the columns by the first wavefront, the rows and the border by
+// the next two.  Call from all 256 threads; a barrier must follow before
+// the pixels read it.
+struct ComposeTile {
+  ComposeCol col[64];
+  ComposeRow row[4];
+  float border;
+};
+
+__device__ inline void compose_tile_stage(const rlevox_raymap& rm, int x0, int y0,
+                                          ComposeTile& t) {
+  const int i = threadIdx.x;
+  if (i < 64)
+    t.col[i] = compose_col(rm, x0 + i);
+  else if (i < 68)
+    t.row[i - 64] = compose_row(rm, y0 + i - 64);
+  else if (i == 128)
+    t.border = compose_border(rm);
+}
+
 __device__ inline void compose_pixel(const rlevox_raymap& rm, const uint32_t* __restrict__ raybuf,
                                      const int32_t* __restrict__ line_clip, int px, int prow,
+                                     ComposeCol col, ComposeRow rw, float border,
                                      uint16_t& out_c, int32_t& out_d) {
   const float RESX = (float)rm.W, RESY = (float)rm.H;
   const int rs = rm.rs;
-  const float border = (RESX - RESY) / (RESX * 2.0f);
   const float vx = rm.vanish[0], vy = rm.vanish[1];
-  const float fcx = (float)px + 0.5f;
-  const float fcy = (float)(rm.H - 1 - prow) + 0.5f;
-  const float scx = fcx / RESX, scy = fcy / RESY;
+  const float scx = col.scx, scy = rw.scy;
   const float scx1 = scx - vx, scy1 = scy - vy;
   const bool upper = !(0.0f < scy1);
   const bool left = !(0.0f < scx1);
-  const float e = fabsf(scy1) - fabsf(scx1) * RESX / RESY;
+  const float e = fabsf(scy1) - col.cterm;
   const bool ostep = !(0.0f < e);
-  const float o2 = (ostep ? fcx : fcy) / RESX;
+  const float o2 = ostep ? scx : rw.ox;  // (ostep ? fcx : fcy) / RESX
   const bool rxg = rm.rot_x_greater_zero != 0.0f;
   float ty, tx;
   if (!ostep) {

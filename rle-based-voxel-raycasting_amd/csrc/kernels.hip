@@ -73,12 +73,17 @@ __global__ __launch_bounds__(256) void k_compose(rlevox_raymap rm, const uint32_
                                                  const int32_t* __restrict__ clip,
                                                  uint16_t* __restrict__ col,
                                                  int32_t* __restrict__ dep) {
-  const int px = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int row = blockIdx.y * 4 + (threadIdx.x >> 6);
+  __shared__ ComposeTile ct;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4;
+  compose_tile_stage(rm, x0, y0, ct);
+  __syncthreads();
+  const int px = x0 + (threadIdx.x & 63);
+  const int row = y0 + (threadIdx.x >> 6);
   if (px >= rm.W || row >= rm.H) return;
   uint16_t c;
   int32_t d;
-  compose_pixel(rm, raybuf, clip, px, row, c, d);
+  compose_pixel(rm, raybuf, clip, px, row, ct.col[threadIdx.x & 63], ct.row[threadIdx.x >> 6],
+                ct.border, c, d);
   col[(size_t)row * rm.W + px] = c;
   dep[(size_t)row * rm.W + px] = d;
 }
