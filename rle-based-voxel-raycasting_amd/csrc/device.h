@@ -15,6 +15,7 @@
 
 #define RLEVOX_HD __host__ __device__
 #include "walk_end.h"
+#include "recip_div.h"
 
 namespace rlevox {
 
@@ -616,12 +617,10 @@ __device__ inline uint32_t gap_word(uint32_t c) {
   return (c & 0xffu) | (1u << 16) | ((c >> 8) << 24);
 }
 
-// floor(num / n) for 0 <= num < 2^13 and 1 <= n <= 26: num/n lies at least
-// 1/(2n) >= 1/52 from an integer once 1/(2n) is added, and the product with
-// the hardware reciprocal (1 ulp) is within 2^13 * 2^-21 = 1/256 of it.
-__device__ inline int small_div(int num, int n) {
-  return (int)(((float)num + 0.5f) * __builtin_amdgcn_rcpf((float)n));
-}
+// The hardware reciprocal (v_rcp_f32, at most 1 ulp off) for recip_div.h.
+struct HwRcp {
+  __device__ float operator()(float x) const { return __builtin_amdgcn_rcpf(x); }
+};
 
 __device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ cin_,
                                     const int32_t* __restrict__ dep_, uint16_t* __restrict__ cout_,
@@ -697,12 +696,7 @@ __device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ c
     uint16_t out = (uint16_t)((ctr.y & 0xffu) | ((ctr.y >> 24) << 8));  // the pixel's colour
     if (dmin < RLEVOX_SKY_DEPTH) {
       const int dm = max(dmin, 1);
-      // floor(78720 / dm): the hardware reciprocal's estimate is within one
-      // of it (relative error below 2^-21 on a value below 78721), then the
-      // integer products settle it
-      int r_q8 = (int)(78720.0f * __builtin_amdgcn_rcpf((float)dm));
-      r_q8 += (r_q8 + 1) * dm <= 78720 ? 1 : 0;
-      r_q8 -= r_q8 * dm > 78720 ? 1 : 0;
+      const int r_q8 = div_78720(dm, HwRcp());  // floor(78720 / dm)
       if (r_q8 >= 42) {
         // tap offsets floor(((2a-5) r + 640) / 1280); the numerators stay
         // above -655360, so a bias of 512*1280 keeps the division unsigned
@@ -732,7 +726,7 @@ __device__ inline void gapfill_tile(int W, int H, const uint16_t* __restrict__ c
           }
           const int n = (int)((acc >> 16) & 0xffu), sum = (int)(acc & 0xffffu);
           // n <= 26, sum + n/2 <= 26 * 255 + 13
-          if (n > 6) out = (uint16_t)((out & 0xff00) | small_div(sum + n / 2, n));
+          if (n > 6) out = (uint16_t)((out & 0xff00) | div_small(sum + n / 2, n, HwRcp()));
         } else {
           out = gapfill_pixel(W, H, cin, dep, x, prow);
         }

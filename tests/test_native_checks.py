@@ -1,7 +1,8 @@
 """Native host checks (CPU): the wavefront kernel's chunked DDA against the
 sequential walk, the early end of a line's walk against the oracle's full
-walk, the batched post passes' tile queues, and the library's host code plus the oracle under AddressSanitizer and
-UndefinedBehaviorSanitizer."""
+walk, the batched post passes' tile queues, the gap-fill's divisions by a
+reciprocal estimate, and the library's host code plus the oracle under
+AddressSanitizer and UndefinedBehaviorSanitizer."""
 import os
 import re
 import subprocess
@@ -84,6 +85,33 @@ def test_walk_end_check_catches_a_wrong_top(tmp_path, plant):
     """The same check with every run top taken `plant` rows too low (a bound
     that is too optimistic) finds steps that still draw."""
     assert _walk_end_run(tmp_path, plant)[3] > 0
+
+
+def _recip_div_run(tmp_path, plant):
+    exe = str(tmp_path / ("recip_div_check_%d" % plant))
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off",
+                    "-DRLEVOX_DIV_PLANT=%d" % plant, os.path.join(NATIVE, "recip_div_check.cpp"),
+                    "-o", exe], check=True)
+    r = subprocess.run([exe], stdout=subprocess.PIPE, text=True)
+    m = re.search(r"recip_div cases (\d+), div_small mismatches (\d+), div_78720 mismatches (\d+)",
+                  r.stdout)
+    assert m, r.stdout
+    return r.returncode, [int(v) for v in m.groups()]
+
+
+def test_gapfill_reciprocal_divisions_are_exact(tmp_path):
+    """The gap-fill's shade average and tap radius from a reciprocal that is
+    up to 2 ulps off equal integer division on their whole input ranges
+    (and 2 M sampled radii above 2^18)."""
+    rc, (cases, small, big) = _recip_div_run(tmp_path, 0)
+    assert rc == 0 and small == 0 and big == 0 and cases > 12000000
+
+
+def test_recip_div_check_catches_planted_mistakes(tmp_path):
+    """Without the half-step bias, or without the upward correction, the same
+    check finds wrong quotients."""
+    assert _recip_div_run(tmp_path, 1)[1][1] > 0
+    assert _recip_div_run(tmp_path, 2)[1][2] > 0
 
 
 def test_host_code_under_sanitizers(tmp_path):
