@@ -21,10 +21,21 @@
 //     next step that acts; only those steps are replayed, in order, using
 //     the precomputed values, and the loop over further elements runs only
 //     when F2 > ymin.
+//     Runs of acting steps of the simple kind (a span drawn from the
+//     horizon up, the top clip untouched, the next element not drawn) are
+//     applied at once by a horizon sweep: the horizon before each step is
+//     a prefix maximum across the lanes (DPP lane moves), and each lane
+//     draws one row of the swept rows.
 //  6. Spans are drawn 64 pixels per pass: lane k applies the per-pixel
 //     increments k times from the span start, exactly as the pixel loop
 //     does (Cuda_Render.h:687), and the line's LDS bitmap is updated from a
 //     ballot of the drawn pixels.
+//  7. The walk ends once no later step can draw (walk_end.h), leaving the
+//     line as the full walk to the far distance does.
+//  8. Two-wavefront form (k_raycast_wave2): one wavefront walks the DDA and
+//     gathers (1-2), the other projects, caches and draws (3-7); chunks pass
+//     between them through a three-slot LDS ring with one workgroup barrier
+//     per chunk.
 #pragma once
 
 #include <type_traits>
@@ -51,12 +62,13 @@ __device__ inline uint64_t uni64(uint64_t v) {
 
 __device__ inline bool lds_bit(const uint32_t* bits, int y) { return (bits[y >> 5] >> (y & 31)) & 1u; }
 
-// LDS hand-off between the lanes of one wavefront.  These kernels run one
-// wavefront per workgroup, and a wavefront's LDS instructions execute in
-// program order, so a write by one lane is seen by a later read of another
-// lane once the compiler keeps both on their side of this point.  Unlike a
-// workgroup barrier, this emits no s_barrier and does not wait for the
-// wavefront's outstanding global loads (the prefetches stay in flight).
+// LDS hand-off between the lanes of one wavefront, on LDS that only this
+// wavefront uses (hand-offs between the two wavefronts of a line go through
+// the ring and its workgroup barrier).  A wavefront's LDS instructions
+// execute in program order, so a write by one lane is seen by a later read
+// of another lane once the compiler keeps both on their side of this point.
+// Unlike a workgroup barrier, this emits no s_barrier and does not wait for
+// the wavefront's outstanding global loads (the prefetches stay in flight).
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
