@@ -133,6 +133,29 @@ def test_c1_mip_and_far_distance(ctx, oracle, c1_scene, c1_dev):
         check_view(ctx, oracle, c1_dev, c1_scene.mips, p, 640, 480, mipd, far)
 
 
+def test_c1_random_pose_sweep(ctx, oracle, c1_scene, c1_dev):
+    """96 seeded random cameras: anywhere over the map (edges and wrap
+    included), pitch -0.6..1.5 (mostly looking down, the method's lower
+    limit |pitch| = 0.01 approached), any heading, eye from inside the
+    terrain to 1500 voxels above it, screens of several aspects, and the
+    mip and far distances varied; every kernel against the oracle."""
+    m0 = c1_scene.mips[0]
+    rng = np.random.default_rng(2024)
+    sizes = [(96, 64), (160, 120), (320, 240), (400, 400), (640, 360), (256, 64), (1024, 576)]
+    dists = [(0, 0), (0, 0), (64, 5000), (512, 30000), (16, 800)]
+    for k in range(96):
+        x = float(rng.uniform(0, m0["sx"])) + 0.17
+        z = float(rng.uniform(0, m0["sz"])) + 0.29
+        alt = float(rng.uniform(-30.0, 1500.0) if k % 3 else rng.uniform(1.0, 40.0))
+        pitch = float(rng.uniform(-0.6, 1.5))
+        pitch = math.copysign(max(abs(pitch), 0.02), pitch)  # the method needs |pitch| >= 0.01
+        yaw = float(rng.uniform(0, 2 * math.pi))
+        p = workloads.pose(x, alt - workloads.surface_row(m0, x, z), z, pitch, yaw)
+        W, H = sizes[k % len(sizes)]
+        mipd, far = dists[k % len(dists)]
+        check_view(ctx, oracle, c1_dev, c1_scene.mips, p, W, H, mipd, far)
+
+
 def test_tiny_wrapping_world(ctx, oracle, tiny_scene, tiny_dev):
     for p in workloads.random_views(tiny_scene.mips[0], 4, seed=5, alt=(2.0, 40.0)):
         check_view(ctx, oracle, tiny_dev, tiny_scene.mips, p, 400, 300)
