@@ -72,8 +72,10 @@ def test_batch_matches_plain(ctx, scenes):
 
 
 def test_batch_matches_oracle(ctx, scenes, oracle):
+    """24 views over three scenes, sizes 320x240 to 1920x1080, in one batch:
+    every view's framebuffer equals the oracle's render."""
     hosts, devs = scenes
-    views = make_views(ctx, hosts, devs, 6, seed=2)
+    views = make_views(ctx, hosts, devs, 24, seed=2)
     ctx.render_batch([v["view"] for v in views])
     for v in views:
         W, H = v["W"], v["H"]
